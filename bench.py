@@ -1,0 +1,143 @@
+"""Benchmark: the 1.0M-element hex8 Taylor bar, FP64, on one MI355X.
+
+One step = one increment of ExplicitSolverCH (time-step control, element
+stress/force pass over every element, nodal predictor + Fint gather +
+corrector + grid update) on the O-grid cylinder of 2,000 hexahedra per
+section x 500 layers (1,000,000 elements, 1,022,541 nodes), copper, linear
+hardening A = 400 MPa, B = 100 MPa, 227 m/s impact, central difference
+(rho_b = 1), Courant time step every 10 increments.  Inputs live in HBM
+before the timed region; the timed region is K steps issued back to back on
+the library's stream, bracketed by device synchronisation.
+
+Prints one JSON line (see the contract in DESIGN.md): value = element-steps/s,
+roofline of the element kernel from HIP events recorded around every element
+launch of the timed region, and a CPU baseline: the oracle (C restatement of
+the reference, OpenMP element loop) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic HBM bytes per element-step (FP64 hex8), from the survey's breakdown:
+#   integration-point history S(6)+p+evp = 8 words x 8 B x 2 (read+write) x 8 points = 1024 B
+#   connectivity 8 x 4 B = 32 B
+#   nodal traffic 276 B per node x (nodes / elements)
+HIST_B, CONN_B, NODE_B, ELEM_NODE_READ_B = 1024.0, 32.0, 276.0, 48.0
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--layers", type=int, default=500)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py: this build runs one domain on one GPU; --gpus %d is not supported "
+                 "(the multi-GPU slab decomposition is described in DESIGN.md, not built)" % args.gpus)
+
+    import numpy as np
+    import redynela_amd as R
+    from redynela_amd import _lib as L
+
+    mesh = R.mesh.ogrid_c1(args.layers)
+    ne, nn = mesh["conn"].shape[0], mesh["x"].shape[0]
+    mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
+    sol = R.ExplicitSolverCH()
+    sol.setDissipation(1.0)
+    dom = R.Domain(mesh, mat, sol)
+    dom.set_option(L.OPT_STORE_FINT, 0)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is optional here: dnlg_synchronize waits for the stream
+        sync = lambda: None  # noqa: E731
+    assert dom.begin(1.0)
+    dom.step_async(args.warmup)
+    dom.synchronize()
+    sync()
+    dom.enable_kernel_timing(True)
+    t0 = time.perf_counter()
+    dom.step_async(args.steps)
+    dom.synchronize()
+    sync()
+    t1 = time.perf_counter()
+    n_el, ms_el = dom.kernel_stats(0)
+    n_nd, ms_nd = dom.kernel_stats(1)
+    dom.enable_kernel_timing(False)
+    elapsed = t1 - t0
+    tinfo = dom.time()
+    value = ne * args.steps / elapsed
+    avg_el = ms_el / max(n_el, 1) * 1e-3
+    avg_nd = ms_nd / max(n_nd, 1) * 1e-3
+    ratio = nn / ne
+    elem_bytes = (HIST_B + CONN_B + ELEM_NODE_READ_B * ratio) * ne
+    step_bytes = (HIST_B + CONN_B + NODE_B * ratio) * ne
+    achieved = elem_bytes / avg_el / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mesh, args)
+
+    out = {
+        "metric": "element-steps/s, 1M hex8 FP64 vs FP32 (err vs FP64 oracle); batched bars/s",
+        "value": value,
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated O-grid mesh, uniform 227 m/s initial velocity)",
+        "config": {"workload": "hex8 O-grid Taylor bar, %d elements (2000/section x %d layers), %d nodes, FP64, "
+                               "copper A=400MPa B=100MPa n=1, 227 m/s, central difference" % (ne, args.layers, nn),
+                   "elements": ne, "nodes": nn, "form": "fp64 bit-exact vs oracle",
+                   "sim_time_end_s": tinfo["t"], "dt_s": tinfo["dt"], "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_elem_hex8 (element stress/force pass)",
+                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": None, "avg_ms": avg_el * 1e3, "launches": n_el,
+                     "algorithmic_bytes_per_launch": elem_bytes},
+        "step_roofline": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
+                          "nodal_kernel_avg_ms": avg_nd * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(mesh, args):
+    """The oracle timed on the host: same mesh and material, bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        import oracle_lib as O
+    except Exception as e:  # pragma: no cover
+        return {"value": None, "unit": "element-steps/s", "cores": 0, "kind": "port", "sample": "unavailable: %s" % e}
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    O.lib().dnlo_set_threads(threads)
+    m = O.Model(mesh, dict(E=115e9, nu=0.3, rho=8930.0, A=400e6, B=100e6, n=1.0), dict(rho_b=1.0), mathmode=1)
+    m.begin(1.0)
+    m.step()
+    t0 = time.perf_counter()
+    for _ in range(args.cpu_steps):
+        m.step()
+    dt = time.perf_counter() - t0
+    ne = mesh["conn"].shape[0]
+    return {"value": ne * args.cpu_steps / dt, "unit": "element-steps/s", "cores": threads, "kind": "port",
+            "sample": "%d increments of the same %d-element mesh after 1 warm-up increment (oracle, OpenMP element "
+                      "loop, sequential element-order assembly), %.1f s" % (args.cpu_steps, ne, dt)}
+
+
+if __name__ == "__main__":
+    main()

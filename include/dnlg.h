@@ -1,0 +1,190 @@
+/*
+ * dnlg.h -- C ABI of libdnlgpu, the MI355X (gfx950) explicit FEM core.
+ *
+ * It replaces, for one domain on one GPU, the hot path of pantale/ReDynELA
+ * (DynELA v1.0):
+ *   - Domain::computeInternalForces()          FE/Domain.h:103, FE/Domain.C:130-738
+ *   - Element::computeInternalForces(Vector&)  FE/Element.h:251, FE/Element.C:164-403
+ *   - Solver::solve(Real upTime) as ExplicitSolverCH::solve
+ *                                              FE/Solver.h:126, FE/ExplicitSolverCH.C:115-278
+ * (FE = Sources/Libraries/finiteElement.)  All calls take plain pointers and
+ * sizes; host buffers belong to the caller, device memory to the handle.
+ * Calls on one handle are serialized by the caller; different handles are
+ * independent.  Every function returns DNLG_OK (0) or a DNLG_E_* code, and
+ * dnlg_last_error() gives the message.
+ */
+#ifndef DNLG_H
+#define DNLG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNLG_ABI_VERSION 1
+
+/* status codes; the numerical ones mirror the reference's fatalError sites */
+#define DNLG_OK 0
+#define DNLG_E_ARG 1            /* bad argument (the reference: fatalError on bad input)          */
+#define DNLG_E_POLAR_NOCONV 2   /* Jacobi polar decomposition > 29 rotations, LA/Tensor2.C:1257    */
+#define DNLG_E_RR_NOCONV 3      /* radial return > 2000 iterations, FE/Element.C:327-330           */
+#define DNLG_E_NEG_DETJ 4       /* negative (2D) or singular (3D) Jacobian, FE/Element2D.C:101-109 */
+#define DNLG_E_HIP 5            /* HIP runtime error                                               */
+#define DNLG_E_STATE 6          /* call out of sequence                                            */
+
+/* element families: the reference's ElQua4nAx (FE/ElQua4nAx.C) and ElHex8n3D (FE/ElHex8n3D.C) */
+#define DNLG_DIM_AXI 2
+#define DNLG_DIM_3D 3
+
+/* mesh: node coordinates (3 per node, z = 0 for axisymmetric) and element
+ * connectivity in the reference element's local node order, 0-based node
+ * indices = the reference's Node::Id after Domain::initSolve (FE/Domain.C:1159) */
+typedef struct
+{
+  int dim;               /* DNLG_DIM_AXI (4-node quads) or DNLG_DIM_3D (8-node hexahedra) */
+  long long n_nodes;
+  long long n_elems;
+  const double *coords;  /* [n_nodes][3] */
+  const int *conn;       /* [n_elems][4 or 8] */
+} dnlg_mesh;
+
+/* Material + IsoHardElastoplastic / IsoHardJohnsonCook (FE/Material.h,
+ * FE/IsoHardElastoplastic.C:171-184, FE/IsoHardJohnsonCook.C:185-198):
+ *   hardening = 0: purely elastic (Material::isYield() == No)
+ *   hardening = 1: sigma_y = A + B evp^n, the law both reference classes evaluate
+ *   hardening = 2: Johnson-Cook with the rate and temperature factors that
+ *                  FE/IsoHardJohnsonCook.h:51-68 documents but the reference
+ *                  never evaluates: (A + B evp^n)[1 + C ln(epsdot/epsdot0)]
+ *                  [1 - ((T-T0)/(Tm-T0))^m], adiabatic heating
+ *                  dT = taylor_quinney sigma_y depsp / (density cp).
+ *                  With C = 0 and taylor_quinney = 0 it is hardening = 1 exactly. */
+typedef struct
+{
+  double young, poisson, density;
+  double A, B, n;
+  int hardening;
+  double C, epsd0, m, T0, Tm, cp, taylor_quinney;
+} dnlg_material;
+
+/* boundary conditions: BoundaryRestrain (FE/Boundary.C:636-666) as a per-node
+ * component mask (bit c = component c held), initial BoundarySpeed
+ * (FE/Boundary.C:378-384) as a per-node velocity, and an optional constant
+ * BoundarySpeed (FE/Boundary.C:478-524): mask + prescribed velocity. */
+typedef struct
+{
+  const unsigned char *restrain;   /* [n_nodes] or NULL */
+  const double *v0;                /* [n_nodes][3] or NULL */
+  const unsigned char *speed_mask; /* [n_nodes] or NULL */
+  const double *speed;             /* [n_nodes][3] or NULL */
+} dnlg_bc;
+
+/* ExplicitSolverCH parameters (FE/ExplicitSolverCH.C:63-84, 336-343,
+ * FE/Solver.C:45-76): spectral radius rho_b (1.0 = central difference),
+ * time-step safety factor, Courant recompute frequency, end time. */
+typedef struct
+{
+  double rho_b;
+  double safety;
+  int ts_freq;
+  double end_time;
+  int form;          /* DNLG_FORM_FP64 or DNLG_FORM_MIXED */
+} dnlg_solver_params;
+
+/* arithmetic forms of the element pass */
+#define DNLG_FORM_FP64 0   /* the reference's arithmetic, operation for operation */
+#define DNLG_FORM_MIXED 1  /* element kinematics, stress update and B^T sigma in FP32; nodal state and time in FP64 */
+
+typedef struct dnlg_handle dnlg_handle;
+
+/* state fields for dnlg_get_field / dnlg_set_field */
+#define DNLG_F_COORDS 0   /* [n_nodes][3] Node::coords                              */
+#define DNLG_F_V 1        /* [n_nodes][3] Current->mat_speed                        */
+#define DNLG_F_A 2        /* [n_nodes][3] Current->dmat_speed                       */
+#define DNLG_F_DU 3       /* [n_nodes][3] Current->delta_disp                       */
+#define DNLG_F_D 4        /* [n_nodes][3] Current->disp                             */
+#define DNLG_F_FINT 5     /* [n_nodes][dim] Domain::Fint, node-major                */
+#define DNLG_F_MASS 6     /* [n_nodes] Node::mass                                   */
+#define DNLG_F_S 7        /* [9][n_elems*nip] IntegrationPoint::S, point-major      */
+#define DNLG_F_EVP 8      /* [n_elems*nip] IntegrationPoint::evp                    */
+#define DNLG_F_P 9        /* [n_elems] IntegrationPoint::p (equal at all points)    */
+#define DNLG_F_EPS 10     /* [9][n_elems*nip] IntegrationPoint::Eps (if tracked)    */
+#define DNLG_F_EPSPLAS 11 /* [9][n_elems*nip] IntegrationPoint::EpsPlas (if tracked)*/
+#define DNLG_F_FE 12      /* [n_elems*nen][dim] element nodal forces of the last pass */
+#define DNLG_F_TEMP 13    /* [n_elems*nip] integration-point temperature (hardening = 2)  */
+/* In DNLG_FORM_MIXED, DNLG_F_S is [6][nip][n_elems] (xx,yy,zz,xy,yz,xz) and
+ * DNLG_F_EVP / DNLG_F_TEMP are [nip][n_elems]; values are widened to double. */
+
+/* options for dnlg_set_option */
+#define DNLG_OPT_KINETIC_ENERGY 1 /* compute Domain::getTotalKineticEnergy every step (default 0) */
+#define DNLG_OPT_STORE_FINT 2     /* keep Domain::Fint in device memory every step (default 1)    */
+#define DNLG_OPT_TRACK_DISP 3     /* keep NodalField::disp (default 1)                            */
+#define DNLG_OPT_TRACK_STRAIN 4   /* keep IntegrationPoint Eps, EpsPlas; set before the first step (default 0) */
+
+int dnlg_abi_version(void);
+
+/* Structure/Domain set-up (Domain::initSolve, FE/Domain.C:1018-1226): copies
+ * the mesh, material, BCs and solver parameters into device memory. */
+int dnlg_create(const dnlg_mesh *mesh, const dnlg_material *mat, const dnlg_bc *bc,
+                const dnlg_solver_params *solver, dnlg_handle **out);
+int dnlg_destroy(dnlg_handle *h);
+int dnlg_set_option(dnlg_handle *h, int option, int value);
+
+/* Domain::computeInternalForces(): stress update of every integration point
+ * with the Current displacement increment and assembly of Domain::Fint.
+ * fint_out ([n_nodes][dim], node-major as the reference's Fint) may be NULL. */
+int dnlg_compute_internal_forces(dnlg_handle *h, double *fint_out);
+
+/* ExplicitSolverCH::solve(upTime) (FE/ExplicitSolverCH.C:115-278) in parts:
+ * dnlg_begin runs the solve() preamble (mass matrix once, internal forces,
+ * forced time step); dnlg_step runs up to nsteps passes of the time loop and
+ * reports whether the loop continues; dnlg_solve does the whole call. */
+int dnlg_begin(dnlg_handle *h, double up_time, int *running);
+int dnlg_step(dnlg_handle *h, int nsteps, int *running);
+int dnlg_solve(dnlg_handle *h, double up_time, long long *steps_done);
+
+/* issue nsteps passes of the time loop on the handle's stream without
+ * waiting (no host synchronisation); dnlg_synchronize waits and checks errors */
+int dnlg_step_async(dnlg_handle *h, int nsteps);
+int dnlg_synchronize(dnlg_handle *h);
+
+/* state exchange with caller-owned host buffers (n = number of doubles) */
+int dnlg_get_field(dnlg_handle *h, int field, double *out, long long n);
+int dnlg_set_field(dnlg_handle *h, int field, const double *in, long long n);
+long long dnlg_field_size(dnlg_handle *h, int field);
+
+/* time scalars: out[0] t, [1] dt, [2] increment, [3] computed time step,
+ * [4] kinetic energy of the last step, [5] alphaM, [6] beta, [7] gamma, [8] OmegaS */
+int dnlg_get_time(dnlg_handle *h, double *out);
+
+/* HIP stream of the handle (hipStream_t), for callers that order work on it */
+void *dnlg_get_stream(dnlg_handle *h);
+
+/* kernel timing: when enabled, HIP events bracket every element-kernel launch
+ * of dnlg_step/dnlg_step_async on the handle's stream; dnlg_kernel_stats
+ * returns the launch count and the summed milliseconds of the element kernel
+ * (which = 0) and of the nodal kernel (which = 1) since the last reset. */
+int dnlg_enable_kernel_timing(dnlg_handle *h, int on);
+int dnlg_kernel_stats(dnlg_handle *h, int which, long long *launches, double *total_ms);
+
+const char *dnlg_last_error(dnlg_handle *h);
+
+/* ---- mesh generators (host only; no GPU call) ----
+ * Axisymmetric Taylor bar, the recipe of Samples/Taylor/Taylor_Axi_Parametric.dyn:
+ * (nr+1)(nz+1) nodes at (i R/nr, j L/nz), nr*nz quads, base (j = 0) restrained
+ * along the axis direction (bit 1), axis nodes (i = 0) restrained along r and z
+ * (bits 0, 2), initial velocity (0, -v0, 0). */
+int dnlg_mesh_axi_bar(int nr, int nz, double radius, double length, double v0, double *coords, int *conn,
+                      unsigned char *restrain, double *vel);
+/* 3D O-grid cylinder: a central n x n block of half-width s_frac*radius and four
+ * ring blocks of n x m elements, nz layers along z; (n+1)^2 + 4 n m nodes and
+ * n^2 + 4 n m elements per layer.  Nodes on z = 0 restrained along z (bit 2),
+ * initial velocity (0, 0, -v0). */
+int dnlg_mesh_ogrid_sizes(int n, int m, int nz, long long *n_nodes, long long *n_elems);
+int dnlg_mesh_ogrid_bar(int n, int m, int nz, double radius, double length, double s_frac, double v0,
+                        double *coords, int *conn, unsigned char *restrain, double *vel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DNLG_H */

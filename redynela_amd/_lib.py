@@ -1,0 +1,108 @@
+"""ctypes binding of libdnlgpu.so (the C ABI declared in include/dnlg.h).
+
+The library is loaded from this package directory only; there is no CPU
+fallback: if it is missing, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdnlgpu.so")
+
+DNLG_OK = 0
+ERRORS = {1: "DNLG_E_ARG", 2: "DNLG_E_POLAR_NOCONV", 3: "DNLG_E_RR_NOCONV", 4: "DNLG_E_NEG_DETJ",
+          5: "DNLG_E_HIP", 6: "DNLG_E_STATE"}
+
+F_COORDS, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_FE, F_TEMP = range(14)
+OPT_KINETIC_ENERGY, OPT_STORE_FINT, OPT_TRACK_DISP, OPT_TRACK_STRAIN = 1, 2, 3, 4
+
+# every symbol include/dnlg.h declares
+EXPORTS = [
+    "dnlg_abi_version", "dnlg_create", "dnlg_destroy", "dnlg_set_option", "dnlg_compute_internal_forces",
+    "dnlg_begin", "dnlg_step", "dnlg_solve", "dnlg_step_async", "dnlg_synchronize", "dnlg_get_field",
+    "dnlg_set_field", "dnlg_field_size", "dnlg_get_time", "dnlg_get_stream", "dnlg_enable_kernel_timing",
+    "dnlg_kernel_stats", "dnlg_last_error", "dnlg_mesh_axi_bar", "dnlg_mesh_ogrid_sizes", "dnlg_mesh_ogrid_bar",
+]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n_nodes", C.c_longlong), ("n_elems", C.c_longlong),
+                ("coords", C.c_void_p), ("conn", C.c_void_p)]
+
+
+class Material(C.Structure):
+    _fields_ = [("young", C.c_double), ("poisson", C.c_double), ("density", C.c_double),
+                ("A", C.c_double), ("B", C.c_double), ("n", C.c_double), ("hardening", C.c_int),
+                ("C", C.c_double), ("epsd0", C.c_double), ("m", C.c_double), ("T0", C.c_double),
+                ("Tm", C.c_double), ("cp", C.c_double), ("taylor_quinney", C.c_double)]
+
+
+class BC(C.Structure):
+    _fields_ = [("restrain", C.c_void_p), ("v0", C.c_void_p), ("speed_mask", C.c_void_p), ("speed", C.c_void_p)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("rho_b", C.c_double), ("safety", C.c_double), ("ts_freq", C.c_int), ("end_time", C.c_double),
+                ("form", C.c_int)]
+
+
+FORM_FP64, FORM_MIXED = 0, 1
+
+
+class DnlgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DnlgError("libdnlgpu.so is not built: run `python -m redynela_amd.build` "
+                            "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        vp, i, ll, d = C.c_void_p, C.c_int, C.c_longlong, C.c_double
+        sig = {
+            "dnlg_abi_version": (i, []),
+            "dnlg_create": (i, [C.POINTER(Mesh), C.POINTER(Material), C.POINTER(BC), C.POINTER(SolverParams),
+                                C.POINTER(vp)]),
+            "dnlg_destroy": (i, [vp]),
+            "dnlg_set_option": (i, [vp, i, i]),
+            "dnlg_compute_internal_forces": (i, [vp, vp]),
+            "dnlg_begin": (i, [vp, d, C.POINTER(i)]),
+            "dnlg_step": (i, [vp, i, C.POINTER(i)]),
+            "dnlg_solve": (i, [vp, d, C.POINTER(ll)]),
+            "dnlg_step_async": (i, [vp, i]),
+            "dnlg_synchronize": (i, [vp]),
+            "dnlg_get_field": (i, [vp, i, vp, ll]),
+            "dnlg_set_field": (i, [vp, i, vp, ll]),
+            "dnlg_field_size": (ll, [vp, i]),
+            "dnlg_get_time": (i, [vp, vp]),
+            "dnlg_get_stream": (vp, [vp]),
+            "dnlg_enable_kernel_timing": (i, [vp, i]),
+            "dnlg_kernel_stats": (i, [vp, i, C.POINTER(ll), C.POINTER(d)]),
+            "dnlg_last_error": (C.c_char_p, [vp]),
+            "dnlg_mesh_axi_bar": (i, [i, i, d, d, d, vp, vp, vp, vp]),
+            "dnlg_mesh_ogrid_sizes": (i, [i, i, i, C.POINTER(ll), C.POINTER(ll)]),
+            "dnlg_mesh_ogrid_bar": (i, [i, i, i, d, d, d, d, vp, vp, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, h=None):
+    if rc != DNLG_OK:
+        msg = lib().dnlg_last_error(h)
+        raise DnlgError("%s: %s" % (ERRORS.get(rc, rc), msg.decode() if msg else ""))
+
+
+def ptr(a):
+    return a.ctypes.data if a is not None else None

@@ -1,0 +1,815 @@
+// C ABI of libdnlgpu: handle life cycle, device memory, step issuing.
+//
+// The handle holds one Domain of the reference (FE/Domain.h:59-132) on one
+// GPU: SoA/AoS device arrays, the node -> (element, local node) CSR used by
+// the deterministic gathers, and the device-resident time-step state.  The
+// call sequence of ExplicitSolverCH::solve (FE/ExplicitSolverCH.C:115-278) is
+// issued as kernels on the handle's stream:
+//   preamble  : [mass once] element pass, Fint gather, Courant + forced dt
+//   each step : [Courant when the previous increment % freq == 0] k_dt
+//               (endStep of the previous increment, computeTimeStep, initStep)
+//               element kernel, fused nodal kernel
+// and a final k_dt at every synchronisation point closes the last increment.
+#include "../../include/dnlg.h"
+#include "dnlg_internal.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace dnl {
+void launch_elem_f64(int dim, const ElemArgs &a, bool speed_bc, hipStream_t s);
+void launch_mass_f64(int dim, const ElemArgs &a, double rho, const double *hn, double *me, hipStream_t s);
+void launch_gather_scalar(const long long *off, const int *idx, const double *src, double *dst, long long nn,
+                          hipStream_t s);
+void launch_gather_fint(int dim, const long long *off, const int *idx, const double *fe, const float *fe32,
+                        double *fint, long long nn, hipStream_t s);
+void launch_elem_f32(int dim, const ElemArgs32 &a, hipStream_t s);
+void launch_nodal_f64(int dim, const NodalArgs &n, bool track_d, bool has_bcs, hipStream_t s);
+void launch_courant_f64(int dim, const ElemArgs &a, double *part, hipStream_t s);
+void launch_dt(StepState *st, const double *part, int npart, int recompute, const double *ke_part, int nke, int mode,
+               hipStream_t s);
+unsigned nodal_blocks(long long nn);
+unsigned courant_blocks(long long ne);
+}  // namespace dnl
+
+using namespace dnl;
+
+struct dnlg_handle
+{
+  int dim = 0, nen = 0, nip = 0;
+  long long nn = 0, ne = 0;
+  hipStream_t stream = nullptr;
+  // device buffers
+  double *x = nullptr, *v = nullptr, *a = nullptr, *du = nullptr, *d = nullptr, *mass = nullptr;
+  double *fint = nullptr, *fe = nullptr, *S = nullptr, *evp = nullptr, *p = nullptr;
+  double *eps = nullptr, *epsplas = nullptr, *me = nullptr, *part = nullptr, *ke_part = nullptr;
+  double *bcs_v = nullptr;
+  int *conn = nullptr, *idx = nullptr;
+  long long *off = nullptr;
+  unsigned char *bc = nullptr, *bcs = nullptr;
+  StepState *st = nullptr;
+  bool has_bcs = false;
+  int form = DNLG_FORM_FP64;
+  double *T = nullptr;                       // FP64 point temperature (hardening 2)
+  float *S32 = nullptr, *evp32 = nullptr, *T32 = nullptr, *p32 = nullptr, *fe32 = nullptr;  // mixed form
+  ElemArgs32 ea32{};
+  // options
+  bool ke_on = false, store_fint = true, track_d = true, track_strain = false;
+  // host-side sequencing
+  bool mass_done = false, pending = false, begun = false;
+  long long inc = 0;
+  int freq = 10;
+  dnlg_material mat{};
+  double hn[4] = {0, 0, 0, 0};
+  ElemArgs ea{};
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_elem, ev_nodal;  // start/stop pairs
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::string err;
+};
+
+static thread_local std::string g_err;
+
+#define DNLG_HIP(h, call)                                                                        \
+  do                                                                                             \
+  {                                                                                              \
+    hipError_t _e = (call);                                                                      \
+    if (_e != hipSuccess)                                                                        \
+    {                                                                                            \
+      (h)->err = std::string("HIP error ") + hipGetErrorString(_e) + " at " #call;               \
+      return DNLG_E_HIP;                                                                         \
+    }                                                                                            \
+  } while (0)
+
+template <typename T>
+static hipError_t dalloc(T **p, size_t n)
+{
+  return hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
+}
+
+extern "C" int dnlg_abi_version(void) { return DNLG_ABI_VERSION; }
+
+extern "C" const char *dnlg_last_error(dnlg_handle *h)
+{
+  if (!h) return g_err.c_str();
+  return h->err.c_str();
+}
+
+static void fill_elem_args(dnlg_handle *h)
+{
+  ElemArgs &a = h->ea;
+  a.conn = h->conn;
+  a.x = h->x;
+  a.du = h->du;
+  a.S = h->S;
+  a.evp = h->evp;
+  a.p = h->p;
+  a.eps = h->track_strain ? h->eps : nullptr;
+  a.epsplas = h->track_strain ? h->epsplas : nullptr;
+  a.fe = h->fe;
+  a.bcs = h->bcs;
+  a.bcs_v = h->bcs_v;
+  a.st = h->st;
+  a.ne = h->ne;
+  const double E = h->mat.young, nu = h->mat.poisson;
+  a.K = E / (3. * (1. - 2. * nu));   // FE/Element.C:232
+  a.G = E / (2. * (1. + nu));        // Material::mu, FE/Material.h:167-172
+  a.A = h->mat.A;
+  a.B = h->mat.B;
+  a.n = h->mat.n;
+  a.nm1 = h->mat.n - 1.;
+  a.yield = h->mat.hardening ? 1 : 0;
+  const dnlg_material &M = h->mat;
+  const double chi = (M.hardening == 2 && M.cp > 0.0) ? M.taylor_quinney / (M.density * M.cp) : 0.0;
+  a.law = M.hardening == 2 ? ((M.C == 0.0 && chi == 0.0) ? 1 : 2) : (M.hardening ? 1 : 0);
+  a.C = M.C;
+  a.epsd0 = M.epsd0;
+  a.m = M.m;
+  a.T0 = M.T0;
+  a.Tm = M.Tm;
+  a.chi_rho_cp = chi;
+  a.T = h->T;
+  ElemArgs32 &b = h->ea32;
+  b.conn = h->conn;
+  b.x = h->x;
+  b.du = h->du;
+  b.S = h->S32;
+  b.evp = h->evp32;
+  b.T = (a.law == 2) ? h->T32 : nullptr;
+  b.p = h->p32;
+  b.fe = h->fe32;
+  b.st = h->st;
+  b.ne = h->ne;
+  b.K = (float)a.K;
+  b.G = (float)a.G;
+  b.A = (float)M.A;
+  b.B = (float)M.B;
+  b.n = (float)M.n;
+  b.C = (float)M.C;
+  b.epsd0 = (float)M.epsd0;
+  b.m = (float)M.m;
+  b.T0 = (float)M.T0;
+  b.Tm = (float)M.Tm;
+  b.chi_rho_cp = (float)chi;
+  b.law = a.law;
+  if (h->dim == 3)
+  {
+    // closed forms of the hex8 tables (FE/ElHex8n3D.C:81-87)
+    a.c0 = (2.0 - sqrt(3.0)) / 12.0;
+    a.c1 = 1.0 / 12.0;
+    a.c2 = (2.0 + sqrt(3.0)) / 12.0;
+    h->hn[0] = (9.0 - 5.0 * sqrt(3.0)) / 36.0;
+    h->hn[1] = (3.0 - sqrt(3.0)) / 36.0;
+    h->hn[2] = (3.0 + sqrt(3.0)) / 36.0;
+    h->hn[3] = (9.0 + 5.0 * sqrt(3.0)) / 36.0;
+  }
+  else
+  {
+    // FE/ElQua4nAx.C:53-57
+    a.c0 = (3.0 - sqrt(3.0)) / 12.0;
+    a.c1 = (3.0 + sqrt(3.0)) / 12.0;
+    a.c2 = 0.0;
+  }
+  a.n0 = (2.0 - sqrt(3.0)) / 6.0;
+  a.n1 = 1.0 / 6.0;
+  a.n2 = (2.0 + sqrt(3.0)) / 6.0;
+  a.n3 = 0.0;
+}
+
+static NodalArgs nodal_args(dnlg_handle *h)
+{
+  NodalArgs n{};
+  n.x = h->x;
+  n.v = h->v;
+  n.a = h->a;
+  n.du = h->du;
+  n.d = h->d;
+  n.mass = h->mass;
+  n.bc = h->bc;
+  n.bcs = h->bcs;
+  n.bcs_v = h->bcs_v;
+  n.off = h->off;
+  n.idx = h->idx;
+  n.fe = h->fe;
+  n.fe32 = h->form == DNLG_FORM_MIXED ? h->fe32 : nullptr;
+  n.fint = h->store_fint ? h->fint : nullptr;
+  n.ke_part = h->ke_on ? h->ke_part : nullptr;
+  n.st = h->st;
+  n.nn = h->nn;
+  return n;
+}
+
+static void free_all(dnlg_handle *h)
+{
+  void *ptrs[] = {h->x, h->v, h->a, h->du, h->d, h->mass, h->fint, h->fe, h->S, h->evp, h->p, h->eps, h->epsplas,
+                  h->me, h->part, h->ke_part, h->bcs_v, h->conn, h->idx, h->off, h->bc, h->bcs, h->st, h->T,
+                  h->S32, h->evp32, h->T32, h->p32, h->fe32};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+extern "C" int dnlg_create(const dnlg_mesh *mesh, const dnlg_material *mat, const dnlg_bc *bc,
+                           const dnlg_solver_params *solver, dnlg_handle **out)
+{
+  if (!out) { g_err = "dnlg_create: null output handle"; return DNLG_E_ARG; }
+  *out = nullptr;
+  if (!mesh || !mat || !solver || !mesh->coords || !mesh->conn || (mesh->dim != 2 && mesh->dim != 3) ||
+      mesh->n_nodes <= 0 || mesh->n_elems <= 0)
+  {
+    g_err = "dnlg_create: bad mesh / material / solver arguments";
+    return DNLG_E_ARG;
+  }
+  if (!(mat->young > 0 && mat->density > 0 && mat->poisson > -1.0 && mat->poisson < 0.5))
+  {
+    g_err = "dnlg_create: material constants out of range";
+    return DNLG_E_ARG;
+  }
+  if (!(solver->rho_b >= 0.0 && solver->rho_b <= 1.0))
+  {
+    g_err = "ExplicitSolverCH::setDissipation: dissipation value out of range [0.0, 1.0]";  // FE/ExplicitSolverCH.C:362-363
+    return DNLG_E_ARG;
+  }
+  if (solver->form != DNLG_FORM_FP64 && solver->form != DNLG_FORM_MIXED)
+  {
+    g_err = "dnlg_create: unknown arithmetic form";
+    return DNLG_E_ARG;
+  }
+  if (mat->hardening < 0 || mat->hardening > 2 ||
+      (mat->hardening == 2 && (mat->C != 0.0 || mat->taylor_quinney != 0.0) &&
+       !(mat->epsd0 > 0.0 && mat->Tm > mat->T0 && mat->cp >= 0.0)))
+  {
+    g_err = "dnlg_create: hardening law parameters out of range";
+    return DNLG_E_ARG;
+  }
+  const int nen = mesh->dim == 3 ? 8 : 4;
+  const long long nn = mesh->n_nodes, ne = mesh->n_elems;
+  if (ne * nen > 2147483647LL || nn > 2147483647LL)
+  {
+    g_err = "dnlg_create: mesh too large for 32-bit element-node indices";
+    return DNLG_E_ARG;
+  }
+  for (long long k = 0; k < ne * nen; k++)
+    if (mesh->conn[k] < 0 || mesh->conn[k] >= nn)
+    {
+      g_err = "dnlg_create: connectivity references a node outside [0, n_nodes)";
+      return DNLG_E_ARG;
+    }
+  dnlg_handle *h = new dnlg_handle();
+  h->dim = mesh->dim;
+  h->nen = nen;
+  h->nip = nen;
+  h->nn = nn;
+  h->ne = ne;
+  h->mat = *mat;
+  h->form = solver->form;
+  h->freq = solver->ts_freq > 0 ? solver->ts_freq : 10;
+  h->has_bcs = bc && bc->speed_mask && bc->speed;
+
+  // node -> (element, local node) CSR, entries in element order (the order
+  // in which Domain::computeInternalForces case 0 adds into Fint)
+  std::vector<long long> off(nn + 1, 0);
+  for (long long k = 0; k < ne * nen; k++) off[mesh->conn[k] + 1]++;
+  for (long long i = 0; i < nn; i++) off[i + 1] += off[i];
+  std::vector<int> idx(ne * nen);
+  {
+    std::vector<long long> cur(off.begin(), off.end() - 1);
+    for (long long e = 0; e < ne; e++)
+      for (int k = 0; k < nen; k++) idx[cur[mesh->conn[e * nen + k]]++] = (int)(e * nen + k);
+  }
+  const long long nipt = ne * h->nip;
+  std::vector<double> zero3(nn * 3, 0.0);
+  std::vector<unsigned char> zb(nn, 0);
+  std::vector<double> evp0(nipt, 1e-10);  // IntegrationPoint::flush, FE/IntegrationPoint.C:117
+
+#define CK(call)                                                                  \
+  do                                                                              \
+  {                                                                               \
+    hipError_t _e = (call);                                                       \
+    if (_e != hipSuccess)                                                         \
+    {                                                                             \
+      g_err = std::string("HIP error ") + hipGetErrorString(_e) + " at " #call;   \
+      free_all(h);                                                                \
+      delete h;                                                                   \
+      return DNLG_E_HIP;                                                          \
+    }                                                                             \
+  } while (0)
+
+  CK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  CK(dalloc(&h->x, nn * 3));
+  CK(dalloc(&h->v, nn * 3));
+  CK(dalloc(&h->a, nn * 3));
+  CK(dalloc(&h->du, nn * 3));
+  CK(dalloc(&h->d, nn * 3));
+  CK(dalloc(&h->mass, nn));
+  CK(dalloc(&h->fint, nn * h->dim));
+  const bool jc = mat->hardening == 2;
+  if (h->form == DNLG_FORM_FP64)
+  {
+    CK(dalloc(&h->fe, ne * nen * h->dim));
+    CK(dalloc(&h->S, nipt * 9));
+    CK(dalloc(&h->evp, nipt));
+    CK(dalloc(&h->p, ne));
+    if (jc)
+    {
+      std::vector<double> t0(nipt, mat->T0);
+      CK(dalloc(&h->T, nipt));
+      CK(hipMemcpy(h->T, t0.data(), sizeof(double) * nipt, hipMemcpyHostToDevice));
+    }
+    CK(hipMemset(h->S, 0, sizeof(double) * nipt * 9));
+    CK(hipMemcpy(h->evp, evp0.data(), sizeof(double) * nipt, hipMemcpyHostToDevice));
+    CK(hipMemset(h->p, 0, sizeof(double) * ne));
+  }
+  else
+  {
+    std::vector<float> e32(nipt, 1e-10f);
+    CK(dalloc(&h->fe32, ne * nen * h->dim));
+    CK(dalloc(&h->S32, nipt * 6));
+    CK(dalloc(&h->evp32, nipt));
+    CK(dalloc(&h->p32, ne));
+    CK(hipMemset(h->S32, 0, sizeof(float) * nipt * 6));
+    CK(hipMemcpy(h->evp32, e32.data(), sizeof(float) * nipt, hipMemcpyHostToDevice));
+    CK(hipMemset(h->p32, 0, sizeof(float) * ne));
+    if (jc)
+    {
+      std::vector<float> t0(nipt, (float)mat->T0);
+      CK(dalloc(&h->T32, nipt));
+      CK(hipMemcpy(h->T32, t0.data(), sizeof(float) * nipt, hipMemcpyHostToDevice));
+    }
+  }
+  CK(dalloc(&h->me, ne * nen));
+  CK(dalloc(&h->part, courant_blocks(ne)));
+  CK(dalloc(&h->ke_part, nodal_blocks(nn)));
+  CK(dalloc(&h->conn, ne * nen));
+  CK(dalloc(&h->idx, ne * nen));
+  CK(dalloc(&h->off, nn + 1));
+  CK(dalloc(&h->bc, nn));
+  CK(dalloc(&h->st, 1));
+  if (h->has_bcs)
+  {
+    CK(dalloc(&h->bcs, nn));
+    CK(dalloc(&h->bcs_v, nn * 3));
+    CK(hipMemcpy(h->bcs, bc->speed_mask, nn, hipMemcpyHostToDevice));
+    CK(hipMemcpy(h->bcs_v, bc->speed, sizeof(double) * nn * 3, hipMemcpyHostToDevice));
+  }
+  CK(hipMemcpy(h->x, mesh->coords, sizeof(double) * nn * 3, hipMemcpyHostToDevice));
+  CK(hipMemcpy(h->v, (bc && bc->v0) ? bc->v0 : zero3.data(), sizeof(double) * nn * 3, hipMemcpyHostToDevice));
+  CK(hipMemcpy(h->a, zero3.data(), sizeof(double) * nn * 3, hipMemcpyHostToDevice));
+  CK(hipMemcpy(h->du, zero3.data(), sizeof(double) * nn * 3, hipMemcpyHostToDevice));
+  CK(hipMemcpy(h->d, zero3.data(), sizeof(double) * nn * 3, hipMemcpyHostToDevice));
+  CK(hipMemset(h->fint, 0, sizeof(double) * nn * h->dim));
+  CK(hipMemcpy(h->conn, mesh->conn, sizeof(int) * ne * nen, hipMemcpyHostToDevice));
+  CK(hipMemcpy(h->idx, idx.data(), sizeof(int) * ne * nen, hipMemcpyHostToDevice));
+  CK(hipMemcpy(h->off, off.data(), sizeof(long long) * (nn + 1), hipMemcpyHostToDevice));
+  CK(hipMemcpy(h->bc, (bc && bc->restrain) ? bc->restrain : zb.data(), nn, hipMemcpyHostToDevice));
+
+  // scalar state: Solver() defaults (FE/Solver.C:45-76) and
+  // ExplicitSolverCH::computeIntegrationParameters (FE/ExplicitSolverCH.C:336-343)
+  StepState s{};
+  const double rb = solver->rho_b;
+  s.alphaM = (2.0 * rb - 1.0) / (1.0 + rb);
+  s.beta = (5.0 - 3.0 * rb) / ((1.0 + rb) * (1.0 + rb) * (2.0 - rb));
+  s.gamma = 1.5 - s.alphaM;
+  s.omegaS = sqrt((12.0 * pow((1 + rb), 3) * (2.0 - rb)) / (10 + 15 * rb - rb * rb + pow(rb, 3) - pow(rb, 4)));
+  s.safety = solver->safety;
+  s.endTime = solver->end_time;
+  s.freq = h->freq;
+  // Element::getSoundSpeed with every nodal density equal to the material's (FE/Element.C:607-621)
+  {
+    const double E = mat->young, Nu = mat->poisson;
+    double ro = 0.;
+    for (int i = 0; i < nen; i++) ro += mat->density;
+    ro /= nen;
+    s.cs = sqrt((E * (1 - Nu)) / (ro * (1 + Nu) * (1 - 2 * Nu)));
+  }
+  CK(hipMemcpy(h->st, &s, sizeof s, hipMemcpyHostToDevice));
+#undef CK
+  fill_elem_args(h);
+  *out = h;
+  return DNLG_OK;
+}
+
+extern "C" int dnlg_destroy(dnlg_handle *h)
+{
+  if (!h) return DNLG_E_ARG;
+  (void)hipStreamSynchronize(h->stream);
+  free_all(h);
+  delete h;
+  return DNLG_OK;
+}
+
+extern "C" int dnlg_set_option(dnlg_handle *h, int option, int value)
+{
+  if (!h) return DNLG_E_ARG;
+  switch (option)
+  {
+  case DNLG_OPT_KINETIC_ENERGY: h->ke_on = value != 0; break;
+  case DNLG_OPT_STORE_FINT: h->store_fint = value != 0; break;
+  case DNLG_OPT_TRACK_DISP: h->track_d = value != 0; break;
+  case DNLG_OPT_TRACK_STRAIN:
+    if (value && h->form != DNLG_FORM_FP64)
+    {
+      h->err = "dnlg_set_option: strain tracking exists in the FP64 form only";
+      return DNLG_E_ARG;
+    }
+    if (value && !h->eps)
+    {
+      const long long nipt = h->ne * h->nip;
+      DNLG_HIP(h, dalloc(&h->eps, nipt * 9));
+      DNLG_HIP(h, dalloc(&h->epsplas, nipt * 9));
+      DNLG_HIP(h, hipMemset(h->eps, 0, sizeof(double) * nipt * 9));
+      DNLG_HIP(h, hipMemset(h->epsplas, 0, sizeof(double) * nipt * 9));
+    }
+    h->track_strain = value != 0;
+    fill_elem_args(h);
+    break;
+  default: h->err = "dnlg_set_option: unknown option"; return DNLG_E_ARG;
+  }
+  return DNLG_OK;
+}
+
+static int read_state(dnlg_handle *h, StepState *s)
+{
+  DNLG_HIP(h, hipMemcpyAsync(s, h->st, sizeof *s, hipMemcpyDeviceToHost, h->stream));
+  DNLG_HIP(h, hipStreamSynchronize(h->stream));
+  return DNLG_OK;
+}
+
+static int check_status(dnlg_handle *h, const StepState &s)
+{
+  if (s.status == ST_OK) return DNLG_OK;
+  char buf[256];
+  const char *what = s.status == ST_POLAR ? "Eigen Value Computation: No convergence (polar decomposition)"
+                     : s.status == ST_RR  ? "Element::computeInternalForces: Too many atempts in plastic procedure"
+                     : s.status == ST_DETJ ? "Negative or singular Jacobian"
+                                           : "device error";
+  snprintf(buf, sizeof buf, "%s (element %d)", what, s.err_elem);
+  h->err = buf;
+  return s.status == ST_POLAR ? DNLG_E_POLAR_NOCONV : s.status == ST_RR ? DNLG_E_RR_NOCONV : DNLG_E_NEG_DETJ;
+}
+
+static void launch_elem(dnlg_handle *h, bool check_running, bool speed)
+{
+  if (h->form == DNLG_FORM_MIXED)
+  {
+    ElemArgs32 b = h->ea32;
+    b.check_running = check_running ? 1 : 0;
+    launch_elem_f32(h->dim, b, h->stream);
+  }
+  else
+  {
+    ElemArgs a = h->ea;
+    a.check_running = check_running ? 1 : 0;
+    launch_elem_f64(h->dim, a, speed, h->stream);
+  }
+}
+
+static hipEvent_t next_event(dnlg_handle *h)
+{
+  if (h->ev_used >= h->ev_pool.size())
+  {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    h->ev_pool.push_back(e);
+  }
+  return h->ev_pool[h->ev_used++];
+}
+
+// Domain::computeInternalForces(): element pass with the Current increment, then Fint
+extern "C" int dnlg_compute_internal_forces(dnlg_handle *h, double *fint_out)
+{
+  if (!h) return DNLG_E_ARG;
+  launch_elem(h, false, false);
+  DNLG_HIP(h, hipGetLastError());
+  launch_gather_fint(h->dim, h->off, h->idx, h->fe, h->form == DNLG_FORM_MIXED ? h->fe32 : nullptr, h->fint, h->nn,
+                     h->stream);
+  DNLG_HIP(h, hipGetLastError());
+  if (fint_out)
+    DNLG_HIP(h, hipMemcpyAsync(fint_out, h->fint, sizeof(double) * h->nn * h->dim, hipMemcpyDeviceToHost, h->stream));
+  StepState s;
+  int rc = read_state(h, &s);
+  if (rc) return rc;
+  return check_status(h, s);
+}
+
+// close the last issued increment: endStep (+ computeTimeStep while running)
+static int finalize(dnlg_handle *h)
+{
+  if (!h->pending) return DNLG_OK;
+  const int rec = (h->inc % h->freq) == 0;
+  if (rec) launch_courant_f64(h->dim, h->ea, h->part, h->stream);
+  launch_dt(h->st, h->part, (int)courant_blocks(h->ne), rec, h->ke_on ? h->ke_part : nullptr,
+            (int)nodal_blocks(h->nn), 1, h->stream);
+  DNLG_HIP(h, hipGetLastError());
+  h->pending = false;
+  return DNLG_OK;
+}
+
+extern "C" int dnlg_begin(dnlg_handle *h, double up_time, int *running)
+{
+  if (!h) return DNLG_E_ARG;
+  int rc = finalize(h);
+  if (rc) return rc;
+  StepState s;
+  rc = read_state(h, &s);
+  if (rc) return rc;
+  // ExplicitSolverCH::solve: upTime, betweenTimeBounds (FE/ExplicitSolverCH.C:121-128)
+  s.upTime = up_time;
+  const double lim = s.endTime < up_time ? s.endTime : up_time;
+  s.running = (s.t >= 0.0) && (s.t < lim);
+  DNLG_HIP(h, hipMemcpyAsync(h->st, &s, sizeof s, hipMemcpyHostToDevice, h->stream));
+  h->inc = s.increment;
+  h->begun = s.running != 0;
+  if (running) *running = s.running;
+  if (!s.running) return DNLG_OK;
+  // computeMassMatrix, once (FE/Domain.C:951)
+  if (!h->mass_done)
+  {
+    launch_mass_f64(h->dim, h->ea, h->mat.density, h->hn, h->me, h->stream);
+    launch_gather_scalar(h->off, h->idx, h->me, h->mass, h->nn, h->stream);
+    DNLG_HIP(h, hipGetLastError());
+    h->mass_done = true;
+  }
+  // computeInternalForces of the preamble (FE/ExplicitSolverCH.C:157)
+  launch_elem(h, false, false);
+  launch_gather_fint(h->dim, h->off, h->idx, h->fe, h->form == DNLG_FORM_MIXED ? h->fe32 : nullptr, h->fint, h->nn,
+                     h->stream);
+  // computeTimeStep(True) (FE/ExplicitSolverCH.C:172)
+  launch_courant_f64(h->dim, h->ea, h->part, h->stream);
+  launch_dt(h->st, h->part, (int)courant_blocks(h->ne), 1, nullptr, 0, 2, h->stream);
+  DNLG_HIP(h, hipGetLastError());
+  rc = read_state(h, &s);
+  if (rc) return rc;
+  return check_status(h, s);
+}
+
+static int issue_steps(dnlg_handle *h, int nsteps)
+{
+  if (!h->begun)
+  {
+    h->err = "dnlg_step: call dnlg_begin first";
+    return DNLG_E_STATE;
+  }
+  const NodalArgs n = nodal_args(h);
+  const int npart = (int)courant_blocks(h->ne), nke = (int)nodal_blocks(h->nn);
+  for (int s = 0; s < nsteps; s++)
+  {
+    int rec = 0;
+    if (h->pending)
+    {
+      rec = (h->inc % h->freq) == 0;
+      if (rec) launch_courant_f64(h->dim, h->ea, h->part, h->stream);
+    }
+    launch_dt(h->st, h->part, npart, rec, h->ke_on ? h->ke_part : nullptr, nke, 0, h->stream);
+    if (h->timing)
+    {
+      hipEvent_t e0 = next_event(h), e1 = next_event(h), e2 = next_event(h), e3 = next_event(h);
+      if (!e0 || !e1 || !e2 || !e3)
+      {
+        h->err = "dnlg_step: hipEventCreate failed";
+        return DNLG_E_HIP;
+      }
+      DNLG_HIP(h, hipEventRecord(e0, h->stream));
+      launch_elem(h, true, h->has_bcs);
+      DNLG_HIP(h, hipEventRecord(e1, h->stream));
+      DNLG_HIP(h, hipEventRecord(e2, h->stream));
+      launch_nodal_f64(h->dim, n, h->track_d, h->has_bcs, h->stream);
+      DNLG_HIP(h, hipEventRecord(e3, h->stream));
+      h->ev_elem.push_back(e0);
+      h->ev_elem.push_back(e1);
+      h->ev_nodal.push_back(e2);
+      h->ev_nodal.push_back(e3);
+    }
+    else
+    {
+      launch_elem(h, true, h->has_bcs);
+      launch_nodal_f64(h->dim, n, h->track_d, h->has_bcs, h->stream);
+    }
+    h->inc++;
+    h->pending = true;
+  }
+  DNLG_HIP(h, hipGetLastError());
+  return DNLG_OK;
+}
+
+extern "C" int dnlg_step_async(dnlg_handle *h, int nsteps)
+{
+  if (!h || nsteps < 0) return DNLG_E_ARG;
+  return issue_steps(h, nsteps);
+}
+
+extern "C" int dnlg_synchronize(dnlg_handle *h)
+{
+  if (!h) return DNLG_E_ARG;
+  int rc = finalize(h);
+  if (rc) return rc;
+  StepState s;
+  rc = read_state(h, &s);
+  if (rc) return rc;
+  h->inc = s.increment;
+  return check_status(h, s);
+}
+
+extern "C" int dnlg_step(dnlg_handle *h, int nsteps, int *running)
+{
+  if (!h || nsteps < 0) return DNLG_E_ARG;
+  int rc = issue_steps(h, nsteps);
+  if (rc) return rc;
+  rc = finalize(h);
+  if (rc) return rc;
+  StepState s;
+  rc = read_state(h, &s);
+  if (rc) return rc;
+  h->inc = s.increment;
+  if (running) *running = s.running;
+  if (!s.running) h->begun = false;
+  return check_status(h, s);
+}
+
+extern "C" int dnlg_solve(dnlg_handle *h, double up_time, long long *steps_done)
+{
+  if (!h) return DNLG_E_ARG;
+  int running = 0;
+  int rc = dnlg_begin(h, up_time, &running);
+  if (rc) return rc;
+  StepState s0;
+  rc = read_state(h, &s0);
+  if (rc) return rc;
+  long long inc0 = s0.increment;
+  // chunks of steps between host checks of the loop condition
+  int chunk = 16;
+  while (running)
+  {
+    rc = dnlg_step(h, chunk, &running);
+    if (rc) return rc;
+    if (chunk < 1024) chunk *= 2;
+  }
+  StepState s;
+  rc = read_state(h, &s);
+  if (rc) return rc;
+  if (steps_done) *steps_done = s.increment - inc0;
+  return DNLG_OK;
+}
+
+extern "C" long long dnlg_field_size(dnlg_handle *h, int field)
+{
+  if (!h) return -1;
+  const long long nipt = h->ne * h->nip;
+  const bool mx = h->form == DNLG_FORM_MIXED;
+  switch (field)
+  {
+  case DNLG_F_COORDS: case DNLG_F_V: case DNLG_F_A: case DNLG_F_DU: case DNLG_F_D: return h->nn * 3;
+  case DNLG_F_FINT: return h->nn * h->dim;
+  case DNLG_F_MASS: return h->nn;
+  case DNLG_F_S: return nipt * (mx ? 6 : 9);
+  case DNLG_F_EVP: return nipt;
+  case DNLG_F_P: return h->ne;
+  case DNLG_F_EPS: case DNLG_F_EPSPLAS: return h->eps ? nipt * 9 : 0;
+  case DNLG_F_FE: return h->ne * h->nen * h->dim;
+  case DNLG_F_TEMP: return (h->T || h->T32) ? nipt : 0;
+  default: return -1;
+  }
+}
+
+static double *field_ptr(dnlg_handle *h, int field)
+{
+  switch (field)
+  {
+  case DNLG_F_COORDS: return h->x;
+  case DNLG_F_V: return h->v;
+  case DNLG_F_A: return h->a;
+  case DNLG_F_DU: return h->du;
+  case DNLG_F_D: return h->d;
+  case DNLG_F_FINT: return h->fint;
+  case DNLG_F_MASS: return h->mass;
+  case DNLG_F_S: return h->S;
+  case DNLG_F_EVP: return h->evp;
+  case DNLG_F_P: return h->p;
+  case DNLG_F_EPS: return h->eps;
+  case DNLG_F_EPSPLAS: return h->epsplas;
+  case DNLG_F_FE: return h->fe;
+  case DNLG_F_TEMP: return h->T;
+  default: return nullptr;
+  }
+}
+
+static float *field_ptr32(dnlg_handle *h, int field)
+{
+  if (h->form != DNLG_FORM_MIXED) return nullptr;
+  switch (field)
+  {
+  case DNLG_F_S: return h->S32;
+  case DNLG_F_EVP: return h->evp32;
+  case DNLG_F_P: return h->p32;
+  case DNLG_F_FE: return h->fe32;
+  case DNLG_F_TEMP: return h->T32;
+  default: return nullptr;
+  }
+}
+
+extern "C" int dnlg_get_field(dnlg_handle *h, int field, double *out, long long n)
+{
+  if (!h || !out) return DNLG_E_ARG;
+  const long long sz = dnlg_field_size(h, field);
+  float *src32 = field_ptr32(h, field);
+  double *src = src32 ? nullptr : field_ptr(h, field);
+  if (sz <= 0 || (!src && !src32) || n != sz)
+  {
+    h->err = "dnlg_get_field: unknown field or size mismatch";
+    return DNLG_E_ARG;
+  }
+  int rc = finalize(h);
+  if (rc) return rc;
+  if (src32)
+  {
+    std::vector<float> tmp(n);
+    DNLG_HIP(h, hipMemcpyAsync(tmp.data(), src32, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+    DNLG_HIP(h, hipStreamSynchronize(h->stream));
+    for (long long k = 0; k < n; k++) out[k] = (double)tmp[k];
+    return DNLG_OK;
+  }
+  DNLG_HIP(h, hipMemcpyAsync(out, src, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+  DNLG_HIP(h, hipStreamSynchronize(h->stream));
+  return DNLG_OK;
+}
+
+extern "C" int dnlg_set_field(dnlg_handle *h, int field, const double *in, long long n)
+{
+  if (!h || !in) return DNLG_E_ARG;
+  const long long sz = dnlg_field_size(h, field);
+  float *dst32 = field_ptr32(h, field);
+  double *dst = dst32 ? nullptr : field_ptr(h, field);
+  if (sz <= 0 || (!dst && !dst32) || n != sz || field == DNLG_F_FE)
+  {
+    h->err = "dnlg_set_field: unknown field or size mismatch";
+    return DNLG_E_ARG;
+  }
+  if (dst32)
+  {
+    std::vector<float> tmp(n);
+    for (long long k = 0; k < n; k++) tmp[k] = (float)in[k];
+    DNLG_HIP(h, hipMemcpyAsync(dst32, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+    DNLG_HIP(h, hipStreamSynchronize(h->stream));
+    return DNLG_OK;
+  }
+  DNLG_HIP(h, hipMemcpyAsync(dst, in, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+  DNLG_HIP(h, hipStreamSynchronize(h->stream));
+  if (field == DNLG_F_MASS) h->mass_done = true;
+  return DNLG_OK;
+}
+
+extern "C" int dnlg_get_time(dnlg_handle *h, double *out)
+{
+  if (!h || !out) return DNLG_E_ARG;
+  int rc = finalize(h);
+  if (rc) return rc;
+  StepState s;
+  rc = read_state(h, &s);
+  if (rc) return rc;
+  out[0] = s.t;
+  out[1] = s.dt;
+  out[2] = (double)s.increment;
+  out[3] = s.computedTimeStep;
+  out[4] = s.ke;
+  out[5] = s.alphaM;
+  out[6] = s.beta;
+  out[7] = s.gamma;
+  out[8] = s.omegaS;
+  return DNLG_OK;
+}
+
+extern "C" void *dnlg_get_stream(dnlg_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+extern "C" int dnlg_enable_kernel_timing(dnlg_handle *h, int on)
+{
+  if (!h) return DNLG_E_ARG;
+  DNLG_HIP(h, hipStreamSynchronize(h->stream));
+  h->timing = on != 0;
+  h->ev_used = 0;
+  h->ev_elem.clear();
+  h->ev_nodal.clear();
+  return DNLG_OK;
+}
+
+extern "C" int dnlg_kernel_stats(dnlg_handle *h, int which, long long *launches, double *total_ms)
+{
+  if (!h || (which != 0 && which != 1)) return DNLG_E_ARG;
+  DNLG_HIP(h, hipStreamSynchronize(h->stream));
+  const std::vector<hipEvent_t> &ev = which == 0 ? h->ev_elem : h->ev_nodal;
+  double tot = 0.0;
+  for (size_t k = 0; k + 1 < ev.size(); k += 2)
+  {
+    float ms = 0.f;
+    DNLG_HIP(h, hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+    tot += ms;
+  }
+  if (launches) *launches = (long long)(ev.size() / 2);
+  if (total_ms) *total_ms = tot;
+  return DNLG_OK;
+}

@@ -1,0 +1,97 @@
+// Internal shared definitions of libdnlgpu: device-resident step state,
+// kernel argument blocks and launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dnl {
+
+// device status codes (mirror include/dnlg.h)
+enum : int { ST_OK = 0, ST_POLAR = 2, ST_RR = 3, ST_DETJ = 4 };
+
+// Time-integration scalars kept on the device so that a step is launch-only.
+// Mirrors Solver / ExplicitSolverCH / Domain scalar state:
+// FE/Solver.C:45-76, FE/ExplicitSolverCH.C:336-343, FE/Domain.h (currentTime).
+struct StepState
+{
+  double t;          // Domain::currentTime
+  double dt;         // Solver::timeStep
+  double upTime;     // Solver::upTime
+  double endTime;    // Solver::endTime
+  double computedTimeStep, tsDate, lastTsDate, lastTimeStep;
+  double alphaM, beta, gamma, omegaS, safety;
+  double cs;         // Element::getSoundSpeed (material constant)
+  double ke;         // Domain::getTotalKineticEnergy of the last step
+  long long increment;
+  int freq;          // computeTimeStepFrequency
+  int running;       // betweenTimeBounds at the end of the last step
+  int status;        // first device error
+  int err_elem;
+  int pending;       // the last issued increment still needs its endStep
+  int pad;
+};
+
+struct ElemArgs
+{
+  const int *conn;       // [ne][nen]
+  const double *x;       // [nn][3]
+  const double *du;      // [nn][3] Current delta_disp
+  double *S;             // [9][ne*nip] deviatoric stress (full tensor)
+  double *evp;           // [ne*nip]
+  double *p;             // [ne] pressure (equal at every IP of an element)
+  double *eps;           // [9][ne*nip] or null
+  double *epsplas;       // [9][ne*nip] or null
+  double *fe;            // [ne*nen][dim] element nodal forces
+  const unsigned char *bcs;  // speed-BC component mask per node or null
+  const double *bcs_v;       // [nn][3]
+  StepState *st;
+  long long ne;
+  double K, G, A, B, n, nm1;
+  int yield;
+  int check_running;
+  // Johnson-Cook rate / thermal extension (law 2); T is the point temperature
+  int law;               // 1: A + B evp^n (reference), 2: Johnson-Cook with rate and adiabatic heating
+  double C, epsd0, m, T0, Tm, chi_rho_cp;
+  double *T;             // [ne*nip] or null
+  // shape constants (closed forms of the 2-point Gauss tables)
+  double c0, c1, c2;     // hex8 dN0 magnitudes (2-r3)/12, 1/12, (2+r3)/12 ; quad4 (3-r3)/12, (3+r3)/12
+  double n0, n1, n2, n3; // quad4 N: (2-r3)/6, 1/6, (2+r3)/6 ; hex8 N not needed here
+};
+
+// mixed-form element pass arguments (FP32 point state)
+struct ElemArgs32
+{
+  const int *conn;
+  const double *x, *du;
+  float *S;      // [6][nip][ne]  (xx, yy, zz, xy, yz, xz), point-major per component
+  float *evp;    // [nip][ne]
+  float *T;      // [nip][ne] temperature (Johnson-Cook thermal softening) or null
+  float *p;      // [ne]
+  float *fe;     // [ne*nen][dim]
+  StepState *st;
+  long long ne;
+  float K, G, A, B, n;
+  // Johnson-Cook rate / thermal extension (law == 2)
+  float C, epsd0, m, T0, Tm, chi_rho_cp;  // chi / (rho cp)
+  int law;       // 0 elastic, 1 A + B evp^n, 2 Johnson-Cook with rate and adiabatic heating
+  int check_running;
+};
+
+// fused nodal step arguments
+struct NodalArgs
+{
+  double *x, *v, *a, *du, *d;
+  const double *mass;
+  const unsigned char *bc, *bcs;
+  const double *bcs_v;
+  const long long *off;
+  const int *idx;
+  const double *fe;        // FP64 element forces (FP64 form)
+  const float *fe32;       // FP32 element forces (mixed form)
+  double *fint;     // optional Domain::Fint copy
+  double *ke_part;  // optional per-block kinetic energy
+  StepState *st;
+  long long nn;
+};
+
+}  // namespace dnl

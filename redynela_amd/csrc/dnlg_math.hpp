@@ -1,0 +1,350 @@
+// Device-side arithmetic of the FP64 element update.
+//
+// Every function keeps the evaluation order of the reference routine it
+// restates (cited per function), and the translation unit that includes this
+// header is compiled with -ffp-contract=off, so each result is the double the
+// reference's own code produces for the same inputs.  The only departure is
+// log/exp/pow: these are deterministic implementations built from IEEE
+// +,-,*,/ (the reference calls glibc), so that the GPU reproduces the same
+// bits on every run and on the CPU.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DNL_HD __host__ __device__ __forceinline__
+
+namespace dnl {
+
+// BT/Math.h:36-46, FE/DefaultValues.h:28-29, LA/Tensor2.C:34
+constexpr double kPI = 3.1415926535897932384626433832795;
+constexpr double kDeuxPI = 6.283185307179586476925286766559;
+constexpr double kSqrt23 = 0.8164965809277260327324280249019;
+constexpr double kSqrt32 = 1.2247448713915890490986420373529;
+constexpr double kRRPrec = 1.0E-8;
+constexpr int kRRMaxIt = 2000;
+constexpr double kPolarPrec = 10.0E-10;
+
+template <typename T> DNL_HD T dabs(T x) { return x < 0 ? -x : x; }  // Abs, BT/Math.h:72
+
+// ---------------------------------------------------------------------------
+// Deterministic log / exp / pow (no reference counterpart; the reference uses
+// glibc).  log: x = 2^k (1+f) with sqrt(1/2) <= 1+f < sqrt(2), s = f/(2+f),
+// log(1+f) = f - hfsq + s (hfsq + R(s^2)), R the Taylor series of
+// 2 atanh(s)/s - 2 truncated after s^24 (fdlibm e_log.c arrangement).
+// exp: Cody-Waite reduction by ln 2, degree-13 Taylor polynomial.
+// ---------------------------------------------------------------------------
+DNL_HD double det_log(double x)
+{
+  if (!(x > 0.0)) return (x == 0.0) ? -__builtin_inf() : __builtin_nan("");
+  if (x == __builtin_inf()) return x;
+  uint64_t b = __builtin_bit_cast(uint64_t, x);
+  int k = 0;
+  if ((b >> 52) == 0) { x = x * 18014398509481984.0; b = __builtin_bit_cast(uint64_t, x); k = -54; }
+  k += (int)((b >> 52) & 0x7ff) - 1023;
+  double m = __builtin_bit_cast(double, (b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+  if (m > 1.4142135623730951) { m = m * 0.5; k += 1; }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  const double R = z * (0.66666666666666666667 +
+                   z * (0.40000000000000000000 +
+                   z * (0.28571428571428571429 +
+                   z * (0.22222222222222222222 +
+                   z * (0.18181818181818181818 +
+                   z * (0.15384615384615384615 +
+                   z * (0.13333333333333333333 +
+                   z * (0.11764705882352941176 +
+                   z * (0.10526315789473684211 +
+                   z * (0.09523809523809523810 +
+                   z * (0.08695652173913043478 +
+                   z * (0.08000000000000000000))))))))))));
+  const double hfsq = 0.5 * f * f;
+  const double dk = (double)k;
+  return dk * 6.93147180369123816490e-01 - ((hfsq - (s * (hfsq + R) + dk * 1.90821492927058770002e-10)) - f);
+}
+
+DNL_HD double det_exp(double y)
+{
+  if (y != y) return y;
+  if (y > 709.78) return __builtin_inf();
+  if (y < -745.2) return 0.0;
+  const double kd = y * 1.44269504088896338700e+00;
+  const int k = (int)(kd < 0.0 ? kd - 0.5 : kd + 0.5);
+  const double dk = (double)k;
+  const double r = (y - dk * 6.93147180369123816490e-01) - dk * 1.90821492927058770002e-10;
+  const double p = 1.0 + r * (1.0 + r * (0.5 + r * (1.66666666666666666667e-01 +
+                   r * (4.16666666666666666667e-02 + r * (8.33333333333333333333e-03 +
+                   r * (1.38888888888888888889e-03 + r * (1.98412698412698412698e-04 +
+                   r * (2.48015873015873015873e-05 + r * (2.75573192239858906526e-06 +
+                   r * (2.75573192239858906526e-07 + r * (2.50521083854417187751e-08 +
+                   r * (2.08767569878680989792e-09 + r * 1.60590438368216145994e-10))))))))))));
+  const int k1 = k / 2, k2 = k - k1;
+  const double s1 = __builtin_bit_cast(double, (uint64_t)(k1 + 1023) << 52);
+  const double s2 = __builtin_bit_cast(double, (uint64_t)(k2 + 1023) << 52);
+  return (p * s1) * s2;
+}
+
+DNL_HD double det_pow(double x, double y)
+{
+  if (y == 0.0) return 1.0;
+  if (y == 1.0) return x;
+  if (x == 1.0) return 1.0;
+  if (!(x > 0.0)) return (x == 0.0) ? (y > 0.0 ? 0.0 : __builtin_inf()) : __builtin_nan("");
+  return det_exp(y * det_log(x));
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 tensors, row-major v[3i+j] (LA/Tensor2.h)
+// ---------------------------------------------------------------------------
+struct T9 { double v[9]; };
+
+// c = a * b, LA/Tensor2.C:360-380
+DNL_HD T9 mul(const T9 &a, const T9 &b)
+{
+  T9 t;
+  t.v[0] = a.v[0] * b.v[0] + a.v[1] * b.v[3] + a.v[2] * b.v[6];
+  t.v[1] = a.v[0] * b.v[1] + a.v[1] * b.v[4] + a.v[2] * b.v[7];
+  t.v[2] = a.v[0] * b.v[2] + a.v[1] * b.v[5] + a.v[2] * b.v[8];
+  t.v[3] = a.v[3] * b.v[0] + a.v[4] * b.v[3] + a.v[5] * b.v[6];
+  t.v[4] = a.v[3] * b.v[1] + a.v[4] * b.v[4] + a.v[5] * b.v[7];
+  t.v[5] = a.v[3] * b.v[2] + a.v[4] * b.v[5] + a.v[5] * b.v[8];
+  t.v[6] = a.v[6] * b.v[0] + a.v[7] * b.v[3] + a.v[8] * b.v[6];
+  t.v[7] = a.v[6] * b.v[1] + a.v[7] * b.v[4] + a.v[8] * b.v[7];
+  t.v[8] = a.v[6] * b.v[2] + a.v[7] * b.v[5] + a.v[8] * b.v[8];
+  return t;
+}
+// c = a * b^T (the product with Transpose(), LA/Tensor2.C:570-586 then 360-380)
+DNL_HD T9 mul_bt(const T9 &a, const T9 &b)
+{
+  T9 t;
+  t.v[0] = a.v[0] * b.v[0] + a.v[1] * b.v[1] + a.v[2] * b.v[2];
+  t.v[1] = a.v[0] * b.v[3] + a.v[1] * b.v[4] + a.v[2] * b.v[5];
+  t.v[2] = a.v[0] * b.v[6] + a.v[1] * b.v[7] + a.v[2] * b.v[8];
+  t.v[3] = a.v[3] * b.v[0] + a.v[4] * b.v[1] + a.v[5] * b.v[2];
+  t.v[4] = a.v[3] * b.v[3] + a.v[4] * b.v[4] + a.v[5] * b.v[5];
+  t.v[5] = a.v[3] * b.v[6] + a.v[4] * b.v[7] + a.v[5] * b.v[8];
+  t.v[6] = a.v[6] * b.v[0] + a.v[7] * b.v[1] + a.v[8] * b.v[2];
+  t.v[7] = a.v[6] * b.v[3] + a.v[7] * b.v[4] + a.v[8] * b.v[5];
+  t.v[8] = a.v[6] * b.v[6] + a.v[7] * b.v[7] + a.v[8] * b.v[8];
+  return t;
+}
+// c = a^T * b
+DNL_HD T9 mul_at(const T9 &a, const T9 &b)
+{
+  T9 t;
+  t.v[0] = a.v[0] * b.v[0] + a.v[3] * b.v[3] + a.v[6] * b.v[6];
+  t.v[1] = a.v[0] * b.v[1] + a.v[3] * b.v[4] + a.v[6] * b.v[7];
+  t.v[2] = a.v[0] * b.v[2] + a.v[3] * b.v[5] + a.v[6] * b.v[8];
+  t.v[3] = a.v[1] * b.v[0] + a.v[4] * b.v[3] + a.v[7] * b.v[6];
+  t.v[4] = a.v[1] * b.v[1] + a.v[4] * b.v[4] + a.v[7] * b.v[7];
+  t.v[5] = a.v[1] * b.v[2] + a.v[4] * b.v[5] + a.v[7] * b.v[8];
+  t.v[6] = a.v[2] * b.v[0] + a.v[5] * b.v[3] + a.v[8] * b.v[6];
+  t.v[7] = a.v[2] * b.v[1] + a.v[5] * b.v[4] + a.v[8] * b.v[7];
+  t.v[8] = a.v[2] * b.v[2] + a.v[5] * b.v[5] + a.v[8] * b.v[8];
+  return t;
+}
+// sum of squares in component order, LA/Tensor2.C:484-511
+DNL_HD double sumsq(const T9 &a)
+{
+  return a.v[0] * a.v[0] + a.v[1] * a.v[1] + a.v[2] * a.v[2] + a.v[3] * a.v[3] + a.v[4] * a.v[4] +
+         a.v[5] * a.v[5] + a.v[6] * a.v[6] + a.v[7] * a.v[7] + a.v[8] * a.v[8];
+}
+
+// Tensor2::polarDecomposeLnU, LA/Tensor2.C:1154-1259.  Returns false when the
+// Jacobi sweep does not converge in 29 rotations.
+DNL_HD bool polar_lnU(const T9 &F, T9 &U, T9 &R)
+{
+  T9 FF = mul_at(F, F);
+  T9 Rot = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+  for (int it = 1; it < 30; it++)
+  {
+    int q = (dabs(FF.v[1]) > dabs(FF.v[2]) ? 1 : 2);
+    const double fq = (q == 1 ? FF.v[1] : FF.v[2]);
+    int p = (dabs(fq) > dabs(FF.v[5]) ? 0 : 1);
+    if (p == 1) q = 2;
+    const double fpq = (p == 0 ? (q == 1 ? FF.v[1] : FF.v[2]) : FF.v[5]);
+    if (dabs(fpq) < kPolarPrec)
+    {
+      T9 u, Ru;
+      const double lam[3] = {FF.v[0], FF.v[4], FF.v[8]};
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+      {
+        const double v0 = Rot.v[c], v1 = Rot.v[3 + c], v2 = Rot.v[6 + c];
+        const double ext[9] = {v0 * v0, v0 * v1, v0 * v2, v1 * v0, v1 * v1, v1 * v2, v2 * v0, v2 * v1, v2 * v2};
+        const double a = det_log(lam[c]) / 2.;
+        const double b = 1. / sqrt(lam[c]);
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+        {
+          if (c == 0) { u.v[k] = a * ext[k]; Ru.v[k] = b * ext[k]; }
+          else { u.v[k] += a * ext[k]; Ru.v[k] += b * ext[k]; }
+        }
+      }
+      U = u;
+      R = mul(F, Ru);
+      return true;
+    }
+    const double fqq = (q == 1 ? FF.v[4] : FF.v[8]);
+    const double fpp = (p == 0 ? FF.v[0] : FF.v[4]);
+    const double alpha = (fqq - fpp) / (2. * fpq);
+    const double t = (alpha != 0. ? 1. / (alpha + sqrt(alpha * alpha + 1.) * (alpha > 0. ? 1. : -1.)) : 1.);
+    const double Ct = 1. / (sqrt(1. + t * t));
+    const double St = t * Ct;
+    // rotation with R(p,q) = S, R(q,p) = -S, R(p,p) = R(q,q) = C
+    T9 r = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+    if (p == 0 && q == 1) { r.v[1] = St; r.v[3] = -St; r.v[0] = Ct; r.v[4] = Ct; }
+    else if (p == 0) { r.v[2] = St; r.v[6] = -St; r.v[0] = Ct; r.v[8] = Ct; }
+    else { r.v[5] = St; r.v[7] = -St; r.v[4] = Ct; r.v[8] = Ct; }
+    const double *R_ = r.v, *F_ = FF.v;
+    // R^T FF R, dense formula of LA/Tensor2.C:1216-1239
+    const double c0 = F_[0] * R_[0] + F_[3] * R_[3] + F_[6] * R_[6];
+    const double c1 = F_[1] * R_[0] + F_[4] * R_[3] + F_[7] * R_[6];
+    const double c2 = F_[2] * R_[0] + F_[5] * R_[3] + F_[8] * R_[6];
+    const double d0 = F_[0] * R_[1] + F_[3] * R_[4] + F_[6] * R_[7];
+    const double d1 = F_[1] * R_[1] + F_[4] * R_[4] + F_[7] * R_[7];
+    const double d2 = F_[2] * R_[1] + F_[5] * R_[4] + F_[8] * R_[7];
+    const double e0 = F_[0] * R_[2] + F_[3] * R_[5] + F_[6] * R_[8];
+    const double e1 = F_[1] * R_[2] + F_[4] * R_[5] + F_[7] * R_[8];
+    const double e2 = F_[2] * R_[2] + F_[5] * R_[5] + F_[8] * R_[8];
+    const double u0 = R_[0] * c0 + R_[3] * c1 + R_[6] * c2;
+    const double u1 = R_[1] * c0 + R_[4] * c1 + c2 * R_[7];
+    const double u2 = R_[2] * c0 + R_[5] * c1 + c2 * R_[8];
+    const double u4 = R_[1] * d0 + R_[4] * d1 + R_[7] * d2;
+    const double u5 = R_[2] * d0 + R_[5] * d1 + d2 * R_[8];
+    const double u8 = R_[2] * e0 + R_[5] * e1 + R_[8] * e2;
+    FF = {{u0, u1, u2, u1, u4, u5, u2, u5, u8}};
+    Rot = mul(Rot, r);
+  }
+  return false;
+}
+
+// Matrix::inverse of a 3x3 (LA/Matrix.C:1653-1723): LAPACK dgetf2 (partial
+// pivoting, reciprocal scaling, rank-1 update), dtrti2 (upper, non-unit),
+// dgetri unblocked column sweep and column swaps; reference BLAS loop orders
+// (LP/Blas/dger.c, dtrmv.c, dgemv.c, dscal.c, idamax.c).  Returns false on an
+// exactly singular pivot.
+DNL_HD bool inv3(const T9 &J, T9 &out)
+{
+  double a[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) a[i][j] = J.v[3 * i + j];
+  int ipiv[3];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+  {
+    int jp = j;
+    double dmax = __builtin_fabs(a[j][j]);
+#pragma unroll
+    for (int i = j + 1; i < 3; i++)
+      if (!(__builtin_fabs(a[i][j]) <= dmax)) { jp = i; dmax = __builtin_fabs(a[i][j]); }
+    ipiv[j] = jp;
+    if (jp != j)
+    {
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+      {
+        // row swap with a register-only select: rows j and jp
+        double tj = a[j][c], tp = (jp == 1 ? a[1][c] : a[2][c]);
+        a[j][c] = tp;
+        if (jp == 1) a[1][c] = tj; else a[2][c] = tj;
+      }
+    }
+    if (a[j][j] != 0.)
+    {
+      if (j < 2)
+      {
+        const double r = 1. / a[j][j];
+#pragma unroll
+        for (int i = j + 1; i < 3; i++) a[i][j] = r * a[i][j];
+      }
+    }
+    else
+      ok = false;
+    if (j < 2)
+#pragma unroll
+      for (int c = j + 1; c < 3; c++)
+        if (a[j][c] != 0.)
+        {
+          const double temp = -1. * a[j][c];
+#pragma unroll
+          for (int i = j + 1; i < 3; i++) a[i][c] = a[i][c] + a[i][j] * temp;
+        }
+  }
+  if (!ok) return false;
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+  {
+    a[j][j] = 1. / a[j][j];
+    const double ajj = -a[j][j];
+#pragma unroll
+    for (int jj = 0; jj < j; jj++)
+      if (a[jj][j] != 0.)
+      {
+        const double temp = a[jj][j];
+#pragma unroll
+        for (int i = 0; i < jj; i++) a[i][j] += temp * a[i][jj];
+        a[jj][j] *= a[jj][jj];
+      }
+#pragma unroll
+    for (int i = 0; i < j; i++) a[i][j] = ajj * a[i][j];
+  }
+  double work[3];
+#pragma unroll
+  for (int j = 2; j >= 0; j--)
+  {
+#pragma unroll
+    for (int i = j + 1; i < 3; i++) { work[i] = a[i][j]; a[i][j] = 0.; }
+    if (j < 2)
+#pragma unroll
+      for (int c = j + 1; c < 3; c++)
+        if (work[c] != 0.)
+        {
+          const double temp = -1. * work[c];
+#pragma unroll
+          for (int i = 0; i < 3; i++) a[i][j] += temp * a[i][c];
+        }
+  }
+#pragma unroll
+  for (int j = 1; j >= 0; j--)
+  {
+    const int jp = ipiv[j];
+    if (jp != j)
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+      {
+        double tj = a[i][j], tp = (jp == 1 ? a[i][1] : a[i][2]);
+        a[i][j] = tp;
+        if (jp == 1) a[i][1] = tj; else a[i][2] = tj;
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) out.v[3 * i + j] = a[i][j];
+  return true;
+}
+
+// one row of dN = dN0 * invJ (dgemm NN order, LP/Blas/dgemm.c: for each
+// column j, C(i,j) = 0 then += (1*B(l,j)) * A(i,l) for l with B(l,j) != 0)
+template <int D>
+DNL_HD void dgemm_row(const double *g, const double *inv, double *out)
+{
+#pragma unroll
+  for (int j = 0; j < D; j++)
+  {
+    double c = 0.;
+#pragma unroll
+    for (int l = 0; l < D; l++)
+    {
+      const double b = inv[l * D + j];
+      c = (b != 0.) ? c + (1.0 * b) * g[l] : c;
+    }
+    out[j] = c;
+  }
+}
+
+}  // namespace dnl

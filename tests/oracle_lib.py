@@ -1,0 +1,143 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) for the tests.
+
+The oracle is the checker: a CPU restatement of the reference algorithm.
+Nothing in the product imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+F_X, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_SIG, F_X0, F_FE = range(15)
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_bp = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        L.dnlo_det_log.restype = C.c_double
+        L.dnlo_det_log.argtypes = [C.c_double]
+        L.dnlo_det_exp.restype = C.c_double
+        L.dnlo_det_exp.argtypes = [C.c_double]
+        L.dnlo_det_pow.restype = C.c_double
+        L.dnlo_det_pow.argtypes = [C.c_double, C.c_double]
+        L.dnlo_polar_lnU.restype = C.c_int
+        L.dnlo_polar_lnU.argtypes = [_dp, _dp, _dp, C.c_int]
+        L.dnlo_inv3.restype = C.c_int
+        L.dnlo_inv3.argtypes = [_dp, _dp]
+        L.dnlo_gemm.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.dnlo_t2_rfi.argtypes = [_dp, _dp, _dp]
+        L.dnlo_t2_J2.restype = C.c_double
+        L.dnlo_t2_J2.argtypes = [_dp]
+        L.dnlo_t2_norme.restype = C.c_double
+        L.dnlo_t2_norme.argtypes = [_dp]
+        L.dnlo_t2_deviator.argtypes = [_dp, _dp]
+        L.dnlo_shape_tables.argtypes = [C.c_int, _dp, _dp]
+        L.dnlo_create.restype = C.c_void_p
+        L.dnlo_create.argtypes = [C.c_int, C.c_long, C.c_long, _dp, _ip, _bp, _dp]
+        L.dnlo_set_material.argtypes = [C.c_void_p] + [C.c_double] * 6 + [C.c_int]
+        L.dnlo_set_solver.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double]
+        L.dnlo_set_mathmode.argtypes = [C.c_void_p, C.c_int]
+        L.dnlo_set_ke.argtypes = [C.c_void_p, C.c_int]
+        L.dnlo_set_speed_bc.argtypes = [C.c_void_p, C.c_long, C.c_int, _dp]
+        L.dnlo_set_threads.argtypes = [C.c_int]
+        L.dnlo_init.restype = C.c_int
+        L.dnlo_init.argtypes = [C.c_void_p]
+        L.dnlo_begin.restype = C.c_int
+        L.dnlo_begin.argtypes = [C.c_void_p, C.c_double]
+        L.dnlo_step.restype = C.c_int
+        L.dnlo_step.argtypes = [C.c_void_p]
+        L.dnlo_solve.restype = C.c_long
+        L.dnlo_solve.argtypes = [C.c_void_p, C.c_double]
+        L.dnlo_compute_internal_forces.restype = C.c_int
+        L.dnlo_compute_internal_forces.argtypes = [C.c_void_p]
+        L.dnlo_compute_internal_matrices.restype = C.c_int
+        L.dnlo_compute_internal_matrices.argtypes = [C.c_void_p]
+        L.dnlo_courant_crit.restype = C.c_double
+        L.dnlo_courant_crit.argtypes = [C.c_void_p]
+        L.dnlo_element_length.restype = C.c_double
+        L.dnlo_element_length.argtypes = [C.c_void_p, C.c_long]
+        L.dnlo_get.restype = C.c_long
+        L.dnlo_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.dnlo_get_time.argtypes = [C.c_void_p, _dp]
+        L.dnlo_set_du.argtypes = [C.c_void_p, _dp]
+        L.dnlo_free.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def polar(F, mathmode=0):
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    U = np.zeros(9)
+    R = np.zeros(9)
+    st = lib().dnlo_polar_lnU(F, U, R, mathmode)
+    return st, U.reshape(3, 3), R.reshape(3, 3)
+
+
+class Model:
+    """An oracle model: mesh + material + solver, stepped like ExplicitSolverCH."""
+
+    def __init__(self, mesh, mat, solver, mathmode=1, ke=False):
+        L = lib()
+        self.dim = mesh["dim"]
+        x0 = np.ascontiguousarray(mesh["x"], dtype=np.float64)
+        conn = np.ascontiguousarray(mesh["conn"], dtype=np.int32)
+        bc = np.ascontiguousarray(mesh["bc"], dtype=np.uint8)
+        v0 = np.ascontiguousarray(mesh["v0"], dtype=np.float64)
+        self.nn = x0.shape[0]
+        self.ne = conn.shape[0]
+        self.h = L.dnlo_create(self.dim, self.nn, self.ne, x0.reshape(-1), conn.reshape(-1), bc, v0.reshape(-1))
+        if not self.h:
+            raise ValueError("oracle: bad model")
+        L.dnlo_set_material(self.h, mat["E"], mat["nu"], mat["rho"], mat["A"], mat["B"], mat["n"],
+                            int(mat.get("yield", 1)))
+        L.dnlo_set_solver(self.h, solver.get("rho_b", 0.8182), solver.get("safety", 0.9),
+                          int(solver.get("freq", 10)), solver.get("end_time", 1e300))
+        L.dnlo_set_mathmode(self.h, mathmode)
+        L.dnlo_set_ke(self.h, 1 if ke else 0)
+        st = L.dnlo_init(self.h)
+        if st:
+            raise RuntimeError("oracle init status %d" % st)
+
+    def get(self, field):
+        L = lib()
+        n = L.dnlo_get(self.h, field, None)
+        out = np.zeros(n)
+        L.dnlo_get(self.h, field, out.ctypes.data)
+        return out
+
+    def time(self):
+        out = np.zeros(9)
+        lib().dnlo_get_time(self.h, out)
+        return dict(t=out[0], dt=out[1], increment=int(out[2]), computed_dt=out[3], ke=out[4],
+                    alphaM=out[5], beta=out[6], gamma=out[7], omegaS=out[8])
+
+    def begin(self, up_time):
+        return lib().dnlo_begin(self.h, up_time)
+
+    def step(self):
+        return lib().dnlo_step(self.h)
+
+    def solve(self, up_time):
+        return lib().dnlo_solve(self.h, up_time)
+
+    def compute_internal_forces(self):
+        return lib().dnlo_compute_internal_forces(self.h)
+
+    def set_du(self, du):
+        lib().dnlo_set_du(self.h, np.ascontiguousarray(du, dtype=np.float64).reshape(-1))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().dnlo_free(self.h)
+            self.h = None
