@@ -41,6 +41,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this build runs one domain on one GPU; --gpus %d is not supported "
@@ -85,9 +86,14 @@ def main():
     step_bytes = (HIST_B + CONN_B + NODE_B * ratio) * ne
     achieved = elem_bytes / avg_el / 1e9
 
+    dom.close()
+    fp32 = None
+    if not args.no_fp32:
+        fp32 = mixed_form(mesh, args, value)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args)
+    traffic = profiled_traffic()
 
     out = {
         "metric": "element-steps/s, 1M hex8 FP64 vs FP32 (err vs FP64 oracle); batched bars/s",
@@ -108,13 +114,52 @@ def main():
                    "sim_time_end_s": tinfo["t"], "dt_s": tinfo["dt"], "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_elem_hex8 (element stress/force pass)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": None, "avg_ms": avg_el * 1e3, "launches": n_el,
+                     "traffic": traffic, "avg_ms": avg_el * 1e3, "launches": n_el,
                      "algorithmic_bytes_per_launch": elem_bytes},
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
                           "nodal_kernel_avg_ms": avg_nd * 1e3},
+        "fp32_form": fp32,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def mixed_form(mesh, args, fp64_value):
+    """The mixed-precision form (FP32 element pass, FP64 nodal state) on the same workload."""
+    import redynela_amd as R
+    mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
+    sol = R.ExplicitSolverCH()
+    sol.setDissipation(1.0)
+    sol.setPrecision("mixed")
+    dom = R.Domain(mesh, mat, sol)
+    from redynela_amd import _lib as L
+    dom.set_option(L.OPT_STORE_FINT, 0)
+    assert dom.begin(1.0)
+    dom.step_async(args.warmup)
+    dom.synchronize()
+    t0 = time.perf_counter()
+    dom.step_async(args.steps)
+    dom.synchronize()
+    el = time.perf_counter() - t0
+    ne = mesh["conn"].shape[0]
+    v = ne * args.steps / el
+    dom.close()
+    return {"value": v, "unit": "element-steps/s", "ms_per_step": el / args.steps * 1e3,
+            "speedup_vs_fp64": v / fp64_value,
+            "note": "same mesh, steps and warm-up; FP32 kinematics / stress / B^T sigma, FP64 nodal state and time"}
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the element kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel)."""
+    p = os.path.join(ROOT, "profiles", "k_elem_hex8_fp64_1M.md")
+    try:
+        for line in open(p):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(mesh, args):

@@ -670,8 +670,10 @@ static double o_radius(const dnlo *m, long e, int pt)
  *   sigma_y = (A + B evp^n) [1 + C ln(epsdot/epsdot0)] theta(T),
  *   rate factor 1 when epsdot < epsdot0, theta = 1 - ((T-T0)/(Tm-T0))^m on
  *   [T0, Tm), 1 below, 0 above; epsdot = sqrt(2/3) gamma / dt.
- * Newton on Strial - sqrt(3/2) 2G gamma - sigma_y = 0 with the reference's
- * initial guess, stop |res/sigma_y| <= 1e-8 and safeguard (FE/Element.C:292-331).
+ * Newton on Strial - sqrt(3/2) 2G gamma - sigma_y = 0 from the reference's
+ * initial guess and with its stop |res/sigma_y| <= 1e-8 (FE/Element.C:292-331),
+ * kept inside a sign bracket [lo, hi] (bisection when a step leaves it): the
+ * rate factor has a kink at epsdot = epsdot0 where plain Newton cycles.
  * Adiabatic heating dT = chi/(rho cp) sigma_y depsp after convergence. */
 static double o_theta(const dnlo *m, double T)
 {
@@ -692,6 +694,9 @@ static int o_stress_jc(dnlo *m, long ip, double *S, double G)
   if (!(Strial > sy0)) return O_OK;
   double h0 = m->n * m->B * o_pow(mm, evp, (m->n - 1.)) * theta;
   double gamma = (nrm - O_SQRT23 * sy0) / (2.0 * G * (1.0 + h0 / (3.0 * G)));
+  /* res(gamma) decreases from res(0) > 0 to res(hi) = -sigma_y < 0 */
+  double lo = 0.0, hi = Strial / (O_SQRT32 * 2.0 * G);
+  if (!(gamma > lo && gamma < hi)) gamma = 0.5 * (lo + hi);
   double sy = sy0;
   int it = 0;
   while (1)
@@ -715,9 +720,9 @@ static int o_stress_jc(dnlo *m, long ip, double *S, double G)
     it++;
     if (O_ABS(res / sy) <= O_RR_PREC) break;
     if (it > O_RR_MAXIT) return O_E_RR;
-    double dg = res / (O_SQRT32 * 2.0 * G + dsy);
-    if (dg < 0 && O_ABS(dg) > gamma) dg = -gamma * 0.9;
-    gamma += dg;
+    if (res > 0.0) lo = gamma; else hi = gamma;
+    double gn = gamma + res / (O_SQRT32 * 2.0 * G + dsy);
+    gamma = (gn > lo && gn < hi) ? gn : 0.5 * (lo + hi);  /* Newton inside the bracket, else bisection */
   }
   double n1[9];
   for (int k = 0; k < 9; k++) n1[k] = S[k] / nrm;

@@ -22,10 +22,9 @@
 namespace dnl {
 void launch_elem_f64(int dim, const ElemArgs &a, bool speed_bc, hipStream_t s);
 void launch_mass_f64(int dim, const ElemArgs &a, double rho, const double *hn, double *me, hipStream_t s);
-void launch_gather_scalar(const long long *off, const int *idx, const double *src, double *dst, long long nn,
-                          hipStream_t s);
-void launch_gather_fint(int dim, const long long *off, const int *idx, const double *fe, const float *fe32,
-                        double *fint, long long nn, hipStream_t s);
+void launch_gather_scalar(const unsigned char *deg, const double *src, double *dst, long long nn, hipStream_t s);
+void launch_gather_fint(int dim, const unsigned char *deg, const double *fe, const float *fe32, double *fint,
+                        long long nn, hipStream_t s);
 void launch_elem_f32(int dim, const ElemArgs32 &a, hipStream_t s);
 void launch_nodal_f64(int dim, const NodalArgs &n, bool track_d, bool has_bcs, hipStream_t s);
 void launch_courant_f64(int dim, const ElemArgs &a, double *part, hipStream_t s);
@@ -47,8 +46,10 @@ struct dnlg_handle
   double *fint = nullptr, *fe = nullptr, *S = nullptr, *evp = nullptr, *p = nullptr;
   double *eps = nullptr, *epsplas = nullptr, *me = nullptr, *part = nullptr, *ke_part = nullptr;
   double *bcs_v = nullptr;
-  int *conn = nullptr, *idx = nullptr;
-  long long *off = nullptr;
+  int *conn = nullptr, *fpos = nullptr;
+  unsigned char *deg = nullptr;
+  int maxdeg = 0;
+  std::vector<int> fpos_host;  // (element, local node) -> slot*nn + node
   unsigned char *bc = nullptr, *bcs = nullptr;
   StepState *st = nullptr;
   bool has_bcs = false;
@@ -112,6 +113,7 @@ static void fill_elem_args(dnlg_handle *h)
   a.eps = h->track_strain ? h->eps : nullptr;
   a.epsplas = h->track_strain ? h->epsplas : nullptr;
   a.fe = h->fe;
+  a.fpos = h->fpos;
   a.bcs = h->bcs;
   a.bcs_v = h->bcs_v;
   a.st = h->st;
@@ -143,6 +145,7 @@ static void fill_elem_args(dnlg_handle *h)
   b.T = (a.law == 2) ? h->T32 : nullptr;
   b.p = h->p32;
   b.fe = h->fe32;
+  b.fpos = h->fpos;
   b.st = h->st;
   b.ne = h->ne;
   b.K = (float)a.K;
@@ -193,8 +196,7 @@ static NodalArgs nodal_args(dnlg_handle *h)
   n.bc = h->bc;
   n.bcs = h->bcs;
   n.bcs_v = h->bcs_v;
-  n.off = h->off;
-  n.idx = h->idx;
+  n.deg = h->deg;
   n.fe = h->fe;
   n.fe32 = h->form == DNLG_FORM_MIXED ? h->fe32 : nullptr;
   n.fint = h->store_fint ? h->fint : nullptr;
@@ -207,7 +209,7 @@ static NodalArgs nodal_args(dnlg_handle *h)
 static void free_all(dnlg_handle *h)
 {
   void *ptrs[] = {h->x, h->v, h->a, h->du, h->d, h->mass, h->fint, h->fe, h->S, h->evp, h->p, h->eps, h->epsplas,
-                  h->me, h->part, h->ke_part, h->bcs_v, h->conn, h->idx, h->off, h->bc, h->bcs, h->st, h->T,
+                  h->me, h->part, h->ke_part, h->bcs_v, h->conn, h->fpos, h->deg, h->bc, h->bcs, h->st, h->T,
                   h->S32, h->evp32, h->T32, h->p32, h->fe32};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
@@ -272,16 +274,31 @@ extern "C" int dnlg_create(const dnlg_mesh *mesh, const dnlg_material *mat, cons
   h->freq = solver->ts_freq > 0 ? solver->ts_freq : 10;
   h->has_bcs = bc && bc->speed_mask && bc->speed;
 
-  // node -> (element, local node) CSR, entries in element order (the order
-  // in which Domain::computeInternalForces case 0 adds into Fint)
-  std::vector<long long> off(nn + 1, 0);
-  for (long long k = 0; k < ne * nen; k++) off[mesh->conn[k] + 1]++;
-  for (long long i = 0; i < nn; i++) off[i + 1] += off[i];
-  std::vector<int> idx(ne * nen);
+  // element force slots: node i's s-th element (in element order, the order in
+  // which Domain::computeInternalForces case 0 adds into Fint) writes its
+  // contribution to slot s; slots are [maxdeg][nn] so a node's gather reads
+  // contiguous memory across a wave
+  std::vector<unsigned char> deg(nn, 0);
+  h->fpos_host.assign(ne * nen, 0);
   {
-    std::vector<long long> cur(off.begin(), off.end() - 1);
+    std::vector<int> cnt(nn, 0);
+    for (long long k = 0; k < ne * nen; k++) cnt[mesh->conn[k]]++;
+    int md = 0;
+    for (long long i = 0; i < nn; i++) md = cnt[i] > md ? cnt[i] : md;
+    if (md > 255 || (long long)md * nn > 2147483647LL)
+    {
+      g_err = "dnlg_create: node degree too large";
+      delete h;
+      return DNLG_E_ARG;
+    }
+    h->maxdeg = md;
     for (long long e = 0; e < ne; e++)
-      for (int k = 0; k < nen; k++) idx[cur[mesh->conn[e * nen + k]]++] = (int)(e * nen + k);
+      for (int k = 0; k < nen; k++)
+      {
+        const int node = mesh->conn[e * nen + k];
+        h->fpos_host[e * nen + k] = (int)((long long)deg[node] * nn + node);
+        deg[node]++;
+      }
   }
   const long long nipt = ne * h->nip;
   std::vector<double> zero3(nn * 3, 0.0);
@@ -312,7 +329,7 @@ extern "C" int dnlg_create(const dnlg_mesh *mesh, const dnlg_material *mat, cons
   const bool jc = mat->hardening == 2;
   if (h->form == DNLG_FORM_FP64)
   {
-    CK(dalloc(&h->fe, ne * nen * h->dim));
+    CK(dalloc(&h->fe, (long long)h->maxdeg * nn * h->dim));
     CK(dalloc(&h->S, nipt * 9));
     CK(dalloc(&h->evp, nipt));
     CK(dalloc(&h->p, ne));
@@ -329,7 +346,7 @@ extern "C" int dnlg_create(const dnlg_mesh *mesh, const dnlg_material *mat, cons
   else
   {
     std::vector<float> e32(nipt, 1e-10f);
-    CK(dalloc(&h->fe32, ne * nen * h->dim));
+    CK(dalloc(&h->fe32, (long long)h->maxdeg * nn * h->dim));
     CK(dalloc(&h->S32, nipt * 6));
     CK(dalloc(&h->evp32, nipt));
     CK(dalloc(&h->p32, ne));
@@ -343,12 +360,12 @@ extern "C" int dnlg_create(const dnlg_mesh *mesh, const dnlg_material *mat, cons
       CK(hipMemcpy(h->T32, t0.data(), sizeof(float) * nipt, hipMemcpyHostToDevice));
     }
   }
-  CK(dalloc(&h->me, ne * nen));
+  CK(dalloc(&h->me, (long long)h->maxdeg * nn));
   CK(dalloc(&h->part, courant_blocks(ne)));
   CK(dalloc(&h->ke_part, nodal_blocks(nn)));
   CK(dalloc(&h->conn, ne * nen));
-  CK(dalloc(&h->idx, ne * nen));
-  CK(dalloc(&h->off, nn + 1));
+  CK(dalloc(&h->fpos, ne * nen));
+  CK(dalloc(&h->deg, nn));
   CK(dalloc(&h->bc, nn));
   CK(dalloc(&h->st, 1));
   if (h->has_bcs)
@@ -365,8 +382,8 @@ extern "C" int dnlg_create(const dnlg_mesh *mesh, const dnlg_material *mat, cons
   CK(hipMemcpy(h->d, zero3.data(), sizeof(double) * nn * 3, hipMemcpyHostToDevice));
   CK(hipMemset(h->fint, 0, sizeof(double) * nn * h->dim));
   CK(hipMemcpy(h->conn, mesh->conn, sizeof(int) * ne * nen, hipMemcpyHostToDevice));
-  CK(hipMemcpy(h->idx, idx.data(), sizeof(int) * ne * nen, hipMemcpyHostToDevice));
-  CK(hipMemcpy(h->off, off.data(), sizeof(long long) * (nn + 1), hipMemcpyHostToDevice));
+  CK(hipMemcpy(h->fpos, h->fpos_host.data(), sizeof(int) * ne * nen, hipMemcpyHostToDevice));
+  CK(hipMemcpy(h->deg, deg.data(), nn, hipMemcpyHostToDevice));
   CK(hipMemcpy(h->bc, (bc && bc->restrain) ? bc->restrain : zb.data(), nn, hipMemcpyHostToDevice));
 
   // scalar state: Solver() defaults (FE/Solver.C:45-76) and
@@ -487,7 +504,7 @@ extern "C" int dnlg_compute_internal_forces(dnlg_handle *h, double *fint_out)
   if (!h) return DNLG_E_ARG;
   launch_elem(h, false, false);
   DNLG_HIP(h, hipGetLastError());
-  launch_gather_fint(h->dim, h->off, h->idx, h->fe, h->form == DNLG_FORM_MIXED ? h->fe32 : nullptr, h->fint, h->nn,
+  launch_gather_fint(h->dim, h->deg, h->fe, h->form == DNLG_FORM_MIXED ? h->fe32 : nullptr, h->fint, h->nn,
                      h->stream);
   DNLG_HIP(h, hipGetLastError());
   if (fint_out)
@@ -532,13 +549,13 @@ extern "C" int dnlg_begin(dnlg_handle *h, double up_time, int *running)
   if (!h->mass_done)
   {
     launch_mass_f64(h->dim, h->ea, h->mat.density, h->hn, h->me, h->stream);
-    launch_gather_scalar(h->off, h->idx, h->me, h->mass, h->nn, h->stream);
+    launch_gather_scalar(h->deg, h->me, h->mass, h->nn, h->stream);
     DNLG_HIP(h, hipGetLastError());
     h->mass_done = true;
   }
   // computeInternalForces of the preamble (FE/ExplicitSolverCH.C:157)
   launch_elem(h, false, false);
-  launch_gather_fint(h->dim, h->off, h->idx, h->fe, h->form == DNLG_FORM_MIXED ? h->fe32 : nullptr, h->fint, h->nn,
+  launch_gather_fint(h->dim, h->deg, h->fe, h->form == DNLG_FORM_MIXED ? h->fe32 : nullptr, h->fint, h->nn,
                      h->stream);
   // computeTimeStep(True) (FE/ExplicitSolverCH.C:172)
   launch_courant_f64(h->dim, h->ea, h->part, h->stream);
@@ -726,6 +743,27 @@ extern "C" int dnlg_get_field(dnlg_handle *h, int field, double *out, long long 
   }
   int rc = finalize(h);
   if (rc) return rc;
+  if (field == DNLG_F_FE)
+  {
+    // slots -> [n_elems*nen][dim] in element order
+    const long long tot = (long long)h->maxdeg * h->nn * h->dim;
+    std::vector<double> all(tot);
+    if (src32)
+    {
+      std::vector<float> tmp(tot);
+      DNLG_HIP(h, hipMemcpyAsync(tmp.data(), src32, sizeof(float) * tot, hipMemcpyDeviceToHost, h->stream));
+      DNLG_HIP(h, hipStreamSynchronize(h->stream));
+      for (long long k = 0; k < tot; k++) all[k] = (double)tmp[k];
+    }
+    else
+    {
+      DNLG_HIP(h, hipMemcpyAsync(all.data(), src, sizeof(double) * tot, hipMemcpyDeviceToHost, h->stream));
+      DNLG_HIP(h, hipStreamSynchronize(h->stream));
+    }
+    for (long long k = 0; k < h->ne * h->nen; k++)
+      for (int c = 0; c < h->dim; c++) out[k * h->dim + c] = all[(long long)h->fpos_host[k] * h->dim + c];
+    return DNLG_OK;
+  }
   if (src32)
   {
     std::vector<float> tmp(n);

@@ -204,7 +204,8 @@ __device__ __forceinline__ bool stress_f32(const ElemArgs32 &a, const float *dE,
   // Newton on the consistency condition Strial - sqrt(3/2) 2G gamma = sigma_y(gamma)
   const float h0 = (a.n == 1.f) ? a.B : a.n * a.B * __powf(evp, a.n - 1.f);
   float gamma = (nrm - k23 * sy0) / (2.f * a.G * (1.f + h0 / (3.f * a.G)));
-  if (!(gamma > 0.f)) gamma = 1e-12f;
+  float lo = 0.f, hi = Strial / (k32 * 2.f * a.G);
+  if (!(gamma > lo && gamma < hi)) gamma = 0.5f * (lo + hi);
   int it = 0;
   for (; it < 100; it++)
   {
@@ -212,9 +213,9 @@ __device__ __forceinline__ bool stress_f32(const ElemArgs32 &a, const float *dE,
     yield_f32(a, evp, gamma, dt_inv, theta, sy, dsy);
     const float res = Strial - k32 * 2.f * a.G * gamma - sy;
     if (fabsf(res / sy) <= 1e-6f) break;
-    float dg = res / (k32 * 2.f * a.G + dsy);
-    if (dg < 0.f && -dg > gamma) dg = -0.9f * gamma;
-    gamma += dg;
+    if (res > 0.f) lo = gamma; else hi = gamma;
+    const float gn = gamma + res / (k32 * 2.f * a.G + dsy);
+    gamma = (gn > lo && gn < hi) ? gn : 0.5f * (lo + hi);
   }
   if (it == 100) return false;
   const float f = 1.f - g2 * gamma / nrm;
@@ -355,11 +356,13 @@ __global__ __launch_bounds__(128) void k_elem_hex8_f32(ElemArgs32 a)
   if (!ok) raise32(st, ST_RR, e);
   const float p = a.p[e] + a.K * (pressure * 0.125f);
   a.p[e] = p;
-  float *fo = a.fe + e * 24;
 #pragma unroll
   for (int k = 0; k < 8; k++)
+  {
+    float *fo = a.fe + (long long)a.fpos[e * 8 + k] * 3;
 #pragma unroll
-    for (int i = 0; i < 3; i++) fo[3 * k + i] = -(Af[k][i] + p * g[k][i]);
+    for (int i = 0; i < 3; i++) fo[i] = -(Af[k][i] + p * g[k][i]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -456,12 +459,12 @@ __global__ __launch_bounds__(128) void k_elem_quad4ax_f32(ElemArgs32 a)
   if (!ok) raise32(st, ST_RR, e);
   const float p = a.p[e] + a.K * (pressure * 0.25f);
   a.p[e] = p;
-  float *fo = a.fe + e * 8;
 #pragma unroll
   for (int k = 0; k < 4; k++)
   {
-    fo[2 * k] = -(Af[k][0] + p * g[k][0]);
-    fo[2 * k + 1] = -(Af[k][1] + p * g[k][1]);
+    float *fo = a.fe + (long long)a.fpos[e * 4 + k] * 2;
+    fo[0] = -(Af[k][0] + p * g[k][0]);
+    fo[1] = -(Af[k][1] + p * g[k][1]);
   }
 }
 

@@ -116,8 +116,9 @@ __device__ __forceinline__ bool stress_update(const ElemArgs &a, const T9 &dEps,
 // extension: FE/IsoHardJohnsonCook.h:51-68 documents the law, the reference
 // evaluates only A + B evp^n).  Newton on the consistency condition
 // Strial - sqrt(3/2) 2G gamma = sigma_y(evp + sqrt(2/3) gamma, epsdot, T),
-// same stop and safeguard as FE/Element.C:304-331; adiabatic heating with the
-// Taylor-Quinney fraction after convergence.
+// same stop as FE/Element.C:304-331, kept inside a sign bracket (bisection
+// when a step leaves it: the rate factor has a kink at epsdot = epsdot0);
+// adiabatic heating with the Taylor-Quinney fraction after convergence.
 __device__ __forceinline__ double jc_theta(const ElemArgs &a, double T)
 {
   if (a.chi_rho_cp == 0.0) return 1.0;
@@ -144,6 +145,8 @@ __device__ __forceinline__ bool stress_update_jc(const ElemArgs &a, const T9 &dE
   if (!(Strial > sy0)) return true;
   const double h0 = a.n * a.B * det_pow(evp, a.nm1) * theta;
   double gamma = (nrm - kSqrt23 * sy0) / (2.0 * a.G * (1.0 + h0 / (3.0 * a.G)));
+  double lo = 0.0, hi = Strial / (kSqrt32 * 2.0 * a.G);
+  if (!(gamma > lo && gamma < hi)) gamma = 0.5 * (lo + hi);
   double sy = sy0;
   int it = 0;
   while (true)
@@ -167,9 +170,9 @@ __device__ __forceinline__ bool stress_update_jc(const ElemArgs &a, const T9 &dE
     it++;
     if (dabs(res / sy) <= kRRPrec) break;
     if (it > kRRMaxIt) return false;
-    double dg = res / (kSqrt32 * 2.0 * a.G + dsy);
-    if (dg < 0 && dabs(dg) > gamma) dg = -gamma * 0.9;
-    gamma += dg;
+    if (res > 0.0) lo = gamma; else hi = gamma;
+    const double gn = gamma + res / (kSqrt32 * 2.0 * a.G + dsy);
+    gamma = (gn > lo && gn < hi) ? gn : 0.5 * (lo + hi);
   }
   T9 n1;
 #pragma unroll
@@ -331,7 +334,7 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
   }
   if (valid)
   {
-    double *fo = a.fe + (e * 8 + L) * 3;
+    double *fo = a.fe + (long long)a.fpos[e * 8 + L] * 3;
     fo[0] = f0; fo[1] = f1; fo[2] = f2;
   }
 }
@@ -483,7 +486,7 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
   }
   if (valid)
   {
-    double *fo = a.fe + (e * 4 + L) * 2;
+    double *fo = a.fe + (long long)a.fpos[e * 4 + L] * 2;
     fo[0] = f0; fo[1] = f1;
   }
 }
@@ -560,34 +563,35 @@ __global__ __launch_bounds__(256) void k_mass_elem(ElemArgs a, double rho, doubl
     }
   }
 #pragma unroll
-  for (int i = 0; i < NEN; i++) me[e * NEN + i] = M[i];
+  for (int i = 0; i < NEN; i++) me[a.fpos[e * NEN + i]] = M[i];
 }
 
 // per-node sum of element contributions in element order (mass assembly,
-// FE/Domain.C:966-990); one value per (element, local node) entry
-__global__ __launch_bounds__(256) void k_gather_scalar(const long long *off, const int *idx, const double *src,
-                                                      double *dst, long long nn)
+// FE/Domain.C:966-990): slot s of node i holds its s-th element in element order
+__global__ __launch_bounds__(256) void k_gather_scalar(const unsigned char *deg, const double *src, double *dst,
+                                                      long long nn)
 {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= nn) return;
   double s = 0.;
-  for (long long k = off[i]; k < off[i + 1]; k++) s += src[idx[k]];
+  const int d = deg[i];
+  for (int k = 0; k < d; k++) s += src[(long long)k * nn + i];
   dst[i] = s;
 }
 
 // Domain::Fint only (the preamble's computeInternalForces): gather in element order
 template <int DIM, typename FT>
-__global__ __launch_bounds__(256) void k_gather_fint(const long long *off, const int *idx, const FT *fe,
-                                                    double *fint, long long nn)
+__global__ __launch_bounds__(256) void k_gather_fint(const unsigned char *deg, const FT *fe, double *fint, long long nn)
 {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   if (i >= nn) return;
   double f[DIM];
 #pragma unroll
   for (int j = 0; j < DIM; j++) f[j] = 0.0;
-  for (long long k = off[i]; k < off[i + 1]; k++)
+  const int d = deg[i];
+  for (int k = 0; k < d; k++)
   {
-    const FT *s = fe + (long long)idx[k] * DIM;
+    const FT *s = fe + ((long long)k * nn + i) * DIM;
 #pragma unroll
     for (int j = 0; j < DIM; j++) f[j] += (double)s[j];
   }
@@ -612,13 +616,37 @@ __device__ __forceinline__ void apply_constant(unsigned char b, unsigned char s,
     if (b & (1 << c)) { du[c] = 0.; v[c] = 0.; a[c] = 0.; ac[c] = 0.; }
 }
 
+// [nn][3] AoS arrays are moved through LDS so that every global load and
+// store of a block covers whole contiguous lines (3 x 256 doubles per field)
+__device__ __forceinline__ void stage_in(const double *g, double *l, long long i0, long long nn)
+{
+  const long long base = 3 * i0, lim = 3 * nn;
+#pragma unroll
+  for (int j = threadIdx.x; j < 768; j += 256) l[j] = (base + j < lim) ? g[base + j] : 0.0;
+}
+__device__ __forceinline__ void stage_out(double *g, const double *l, long long i0, long long nn)
+{
+  const long long base = 3 * i0, lim = 3 * nn;
+#pragma unroll
+  for (int j = threadIdx.x; j < 768; j += 256)
+    if (base + j < lim) g[base + j] = l[j];
+}
+
 template <int DIM, bool TRACK_D, bool HAS_BCS, bool FE32>
 __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
 {
   StepState *st = n.st;
+  __shared__ double sx[768], sv[768], sa[768], sd[768], su[768];
   __shared__ double red[256];
   if (!st->running) return;
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long i0 = (long long)blockIdx.x * 256;
+  const long long i = i0 + threadIdx.x;
+  const int t3 = 3 * threadIdx.x;
+  stage_in(n.x, sx, i0, n.nn);
+  stage_in(n.v, sv, i0, n.nn);
+  stage_in(n.a, sa, i0, n.nn);
+  if (TRACK_D) stage_in(n.d, sd, i0, n.nn);
+  __syncthreads();
   double ke = 0.0;
   if (i < n.nn)
   {
@@ -626,7 +654,7 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
     const double s1 = dt * (0.5 - beta), s2 = dt * (1 - gam), s3 = (dt * gam), s4 = ((dt * dt) * beta);
     double ac[3], vc[3], dun[3], vn[3], an[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) { ac[c] = n.a[3 * i + c]; vc[c] = n.v[3 * i + c]; }
+    for (int c = 0; c < 3; c++) { ac[c] = sa[t3 + c]; vc[c] = sv[t3 + c]; }
     // ExplicitSolverCH::computePredictions, FE/ExplicitSolverCH.C:408-424
 #pragma unroll
     for (int c = 0; c < 3; c++)
@@ -650,19 +678,19 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
     double f[DIM];
 #pragma unroll
     for (int j = 0; j < DIM; j++) f[j] = 0.0;
-    for (long long k = n.off[i]; k < n.off[i + 1]; k++)
+    const int dg = n.deg[i];
+    for (int k = 0; k < dg; k++)
     {
+      const long long q = ((long long)k * n.nn + i) * DIM;
       if (FE32)
       {
-        const float *src = n.fe32 + (long long)n.idx[k] * DIM;
 #pragma unroll
-        for (int j = 0; j < DIM; j++) f[j] += (double)src[j];
+        for (int j = 0; j < DIM; j++) f[j] += (double)n.fe32[q + j];
       }
       else
       {
-        const double *src = n.fe + (long long)n.idx[k] * DIM;
 #pragma unroll
-        for (int j = 0; j < DIM; j++) f[j] += src[j];
+        for (int j = 0; j < DIM; j++) f[j] += n.fe[q + j];
       }
     }
     if (n.fint)
@@ -686,14 +714,20 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
 #pragma unroll
     for (int c = 0; c < 3; c++)
     {
-      n.a[3 * i + c] = an[c];
-      n.v[3 * i + c] = vn[c];
-      n.du[3 * i + c] = dun[c];
-      if (TRACK_D) n.d[3 * i + c] = n.d[3 * i + c] + dun[c];
-      n.x[3 * i + c] = n.x[3 * i + c] + dun[c];  // lagMotion::updateNode, FE/lagMotion.C:52-57
+      sa[t3 + c] = an[c];
+      sv[t3 + c] = vn[c];
+      su[t3 + c] = dun[c];
+      if (TRACK_D) sd[t3 + c] = sd[t3 + c] + dun[c];
+      sx[t3 + c] = sx[t3 + c] + dun[c];  // lagMotion::updateNode, FE/lagMotion.C:52-57
     }
     if (n.ke_part) ke = .5 * m * (vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]);
   }
+  __syncthreads();
+  stage_out(n.x, sx, i0, n.nn);
+  stage_out(n.v, sv, i0, n.nn);
+  stage_out(n.a, sa, i0, n.nn);
+  stage_out(n.du, su, i0, n.nn);
+  if (TRACK_D) stage_out(n.d, sd, i0, n.nn);
   if (n.ke_part)
   {
     red[threadIdx.x] = ke;
@@ -910,26 +944,25 @@ void launch_mass_f64(int dim, const ElemArgs &a, double rho, const double *hn, d
   else hipLaunchKernelGGL(k_mass_elem<2>, dim3(grid), dim3(256), 0, s, a, rho, hn[0], hn[1], hn[2], hn[3], me);
 }
 
-void launch_gather_scalar(const long long *off, const int *idx, const double *src, double *dst, long long nn,
-                          hipStream_t s)
+void launch_gather_scalar(const unsigned char *deg, const double *src, double *dst, long long nn, hipStream_t s)
 {
   const unsigned grid = (unsigned)((nn + 255) / 256);
-  hipLaunchKernelGGL(k_gather_scalar, dim3(grid), dim3(256), 0, s, off, idx, src, dst, nn);
+  hipLaunchKernelGGL(k_gather_scalar, dim3(grid), dim3(256), 0, s, deg, src, dst, nn);
 }
 
-void launch_gather_fint(int dim, const long long *off, const int *idx, const double *fe, const float *fe32,
-                        double *fint, long long nn, hipStream_t s)
+void launch_gather_fint(int dim, const unsigned char *deg, const double *fe, const float *fe32, double *fint,
+                        long long nn, hipStream_t s)
 {
   const unsigned grid = (unsigned)((nn + 255) / 256);
   if (fe32)
   {
-    if (dim == 3) hipLaunchKernelGGL((k_gather_fint<3, float>), dim3(grid), dim3(256), 0, s, off, idx, fe32, fint, nn);
-    else hipLaunchKernelGGL((k_gather_fint<2, float>), dim3(grid), dim3(256), 0, s, off, idx, fe32, fint, nn);
+    if (dim == 3) hipLaunchKernelGGL((k_gather_fint<3, float>), dim3(grid), dim3(256), 0, s, deg, fe32, fint, nn);
+    else hipLaunchKernelGGL((k_gather_fint<2, float>), dim3(grid), dim3(256), 0, s, deg, fe32, fint, nn);
   }
   else
   {
-    if (dim == 3) hipLaunchKernelGGL((k_gather_fint<3, double>), dim3(grid), dim3(256), 0, s, off, idx, fe, fint, nn);
-    else hipLaunchKernelGGL((k_gather_fint<2, double>), dim3(grid), dim3(256), 0, s, off, idx, fe, fint, nn);
+    if (dim == 3) hipLaunchKernelGGL((k_gather_fint<3, double>), dim3(grid), dim3(256), 0, s, deg, fe, fint, nn);
+    else hipLaunchKernelGGL((k_gather_fint<2, double>), dim3(grid), dim3(256), 0, s, deg, fe, fint, nn);
   }
 }
 

@@ -41,7 +41,8 @@ struct ElemArgs
   double *p;             // [ne] pressure (equal at every IP of an element)
   double *eps;           // [9][ne*nip] or null
   double *epsplas;       // [9][ne*nip] or null
-  double *fe;            // [ne*nen][dim] element nodal forces
+  double *fe;            // [maxdeg][nn][dim] element nodal forces, slot-major per node (ELL)
+  const int *fpos;       // [ne*nen] slot*nn + node of each (element, local node)
   const unsigned char *bcs;  // speed-BC component mask per node or null
   const double *bcs_v;       // [nn][3]
   StepState *st;
@@ -67,7 +68,8 @@ struct ElemArgs32
   float *evp;    // [nip][ne]
   float *T;      // [nip][ne] temperature (Johnson-Cook thermal softening) or null
   float *p;      // [ne]
-  float *fe;     // [ne*nen][dim]
+  float *fe;     // [maxdeg][nn][dim] (ELL, see ElemArgs::fe)
+  const int *fpos;
   StepState *st;
   long long ne;
   float K, G, A, B, n;
@@ -84,8 +86,7 @@ struct NodalArgs
   const double *mass;
   const unsigned char *bc, *bcs;
   const double *bcs_v;
-  const long long *off;
-  const int *idx;
+  const unsigned char *deg; // elements per node; slot s < deg holds the s-th element in element order
   const double *fe;        // FP64 element forces (FP64 form)
   const float *fe32;       // FP32 element forces (mixed form)
   double *fint;     // optional Domain::Fint copy

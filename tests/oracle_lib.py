@@ -11,7 +11,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
-F_X, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_SIG, F_X0, F_FE = range(15)
+F_X, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_SIG, F_X0, F_FE, F_TEMP = range(16)
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -48,6 +48,7 @@ def lib():
         L.dnlo_set_material.argtypes = [C.c_void_p] + [C.c_double] * 6 + [C.c_int]
         L.dnlo_set_solver.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double]
         L.dnlo_set_mathmode.argtypes = [C.c_void_p, C.c_int]
+        L.dnlo_set_johnson_cook.argtypes = [C.c_void_p] + [C.c_double] * 7
         L.dnlo_set_ke.argtypes = [C.c_void_p, C.c_int]
         L.dnlo_set_speed_bc.argtypes = [C.c_void_p, C.c_long, C.c_int, _dp]
         L.dnlo_set_threads.argtypes = [C.c_int]
@@ -104,6 +105,9 @@ class Model:
         L.dnlo_set_solver(self.h, solver.get("rho_b", 0.8182), solver.get("safety", 0.9),
                           int(solver.get("freq", 10)), solver.get("end_time", 1e300))
         L.dnlo_set_mathmode(self.h, mathmode)
+        if "jc" in mat:  # Johnson-Cook rate / thermal extension
+            j = mat["jc"]
+            L.dnlo_set_johnson_cook(self.h, j["C"], j["epsd0"], j["m"], j["T0"], j["Tm"], j["cp"], j["tq"])
         L.dnlo_set_ke(self.h, 1 if ke else 0)
         st = L.dnlo_init(self.h)
         if st:
@@ -113,7 +117,8 @@ class Model:
         L = lib()
         n = L.dnlo_get(self.h, field, None)
         out = np.zeros(n)
-        L.dnlo_get(self.h, field, out.ctypes.data)
+        if n:
+            L.dnlo_get(self.h, field, out.ctypes.data)
         return out
 
     def time(self):

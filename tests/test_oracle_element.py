@@ -222,3 +222,48 @@ def test_internal_forces_sum_to_zero_3d():
     m.compute_internal_forces()
     F = m.get(O.F_FINT).reshape(-1, 3)
     assert np.max(np.abs(F.sum(axis=0))) < 1e-11 * np.abs(F).max()
+
+
+def _jc_model(C=0.025, tq=0.9, n=0.31):
+    mat = dict(E=115e9, nu=0.3, rho=8930.0, A=90e6, B=292e6, n=n,
+               jc=dict(C=C, epsd0=1.0, m=1.09, T0=293.0, Tm=1356.0, cp=383.0, tq=tq))
+    return O.Model(cube(1e-3), mat, dict(rho_b=1.0)), mat
+
+
+def test_johnson_cook_extension_lands_on_its_yield_surface():
+    """One uniaxial-strain increment at a known rate: the returned state satisfies
+    sqrt(3/2)|S| = (A + B evp^n)(1 + C ln(depsp/dt/epsd0)) theta(T0) and the point
+    heats by chi sigma_y depsp / (rho cp)."""
+    m, mat = _jc_model()
+    j = mat["jc"]
+    L = O.lib()
+    assert m.begin(1.0) == 1                # sets a Courant time step
+    dt = m.time()["dt"]
+    du = np.zeros((8, 3))
+    du[:, 0] = 1.5e-2 * cube(1e-3)["x"][:, 0]
+    m.set_du(du)
+    assert m.compute_internal_forces() == 0
+    S = m.get(O.F_S).reshape(8, 3, 3)
+    evp = m.get(O.F_EVP)
+    T = m.get(O.F_TEMP)
+    for q in range(8):
+        dep = evp[q] - 1e-10
+        assert dep > 0
+        rate = 1.0 + j["C"] * np.log(dep / dt / j["epsd0"])
+        sy = (mat["A"] + mat["B"] * evp[q] ** mat["n"]) * rate  # theta(T0) = 1
+        eq = np.sqrt(1.5) * np.linalg.norm(S[q])
+        assert abs(eq - sy) < 1e-8 * sy
+        assert abs((T[q] - 293.0) - j["tq"] * sy * dep / (8930.0 * 383.0)) < 1e-9 * (T[q] - 293.0)
+
+
+def test_johnson_cook_extension_without_rate_and_heat_is_bitwise_the_reference_law():
+    a, _ = _jc_model(C=0.0, tq=0.0)
+    mat = dict(E=115e9, nu=0.3, rho=8930.0, A=90e6, B=292e6, n=0.31)
+    b = O.Model(cube(1e-3), mat, dict(rho_b=1.0))
+    du = np.zeros((8, 3))
+    du[:, 0] = 1.5e-2 * cube(1e-3)["x"][:, 0]
+    for m in (a, b):
+        m.set_du(du)
+        m.compute_internal_forces()
+    assert np.array_equal(a.get(O.F_S), b.get(O.F_S))
+    assert np.array_equal(a.get(O.F_EVP), b.get(O.F_EVP))
