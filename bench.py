@@ -42,6 +42,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this build runs one domain on one GPU; --gpus %d is not supported "
@@ -90,6 +91,9 @@ def main():
     fp32 = None
     if not args.no_fp32:
         fp32 = mixed_form(mesh, args, value)
+    batched = None
+    if not args.no_batch:
+        batched = {form: batched_bars(form) for form in ("fp64", "mixed")}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args)
@@ -119,6 +123,7 @@ def main():
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
                           "nodal_kernel_avg_ms": avg_nd * 1e3},
         "fp32_form": fp32,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -147,6 +152,34 @@ def mixed_form(mesh, args, fp64_value):
     return {"value": v, "unit": "element-steps/s", "ms_per_step": el / args.steps * 1e3,
             "speedup_vs_fp64": v / fp64_value,
             "note": "same mesh, steps and warm-up; FP32 kinematics / stress / B^T sigma, FP64 nodal state and time"}
+
+
+def batched_bars(form, n=512, t_end=40e-6):
+    """The batched workload: 512 seeded Taylor bars (nr ~ U{5..20}, nz = 5 nr,
+    125..2,000 quads, drawn speed and hardening) advanced to 40 us by one
+    persistent launch; bars/s from the launch's HIP events, per-bar device
+    wall time (first to last instruction of the bar's workgroup)."""
+    import numpy as np
+    import redynela_amd as R
+    meshes, mats = R.batch.taylor_batch(n)
+    sol = R.ExplicitSolverCH()
+    sol.setDissipation(1.0)
+    sol.setPrecision(form)
+    warm = R.Batch(meshes[:8], mats[:8], sol)  # loads the code object
+    warm.solve(t_end)
+    warm.close()
+    b = R.Batch(meshes, mats, sol)
+    t0 = time.perf_counter()
+    wall_us, ms = b.solve(t_end)
+    host_s = time.perf_counter() - t0
+    steps = np.array([b.time(i)["increment"] for i in range(n)])
+    ne = np.array([m["conn"].shape[0] for m in meshes])
+    b.close()
+    return {"bars": n, "bars_per_s": n / (ms * 1e-3), "launch_ms": ms, "host_solve_s": host_s,
+            "p50_us_per_bar": float(np.percentile(wall_us, 50)), "p99_us_per_bar": float(np.percentile(wall_us, 99)),
+            "max_us_per_bar": float(wall_us.max()), "element_steps": int((ne * steps).sum()),
+            "element_steps_per_s": float((ne * steps).sum() / (ms * 1e-3)),
+            "increments_min_max": [int(steps.min()), int(steps.max())], "elements_total": int(ne.sum())}
 
 
 def profiled_traffic():

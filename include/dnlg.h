@@ -184,6 +184,37 @@ int dnlg_mesh_ogrid_sizes(int n, int m, int nz, long long *n_nodes, long long *n
 int dnlg_mesh_ogrid_bar(int n, int m, int nz, double radius, double length, double s_frac, double v0,
                         double *coords, int *conn, unsigned char *restrain, double *vel);
 
+/* ---- batched mode: many independent axisymmetric domains, one launch ----
+ * Each domain is the whole of one ExplicitSolverCH::solve(upTime)
+ * (FE/ExplicitSolverCH.C:115-278) with its own mesh, hardening parameters,
+ * initial velocity, time step and increment count; the reference runs one
+ * Structure per process (FE/Structure.C:809-897), so a caller replaces a loop
+ * of those runs with one dnlg_batch_solve.  Every domain is a quad4
+ * axisymmetric mesh (dim 2); E, nu and density are shared; hardening 0 or 1;
+ * no speed BCs.  Domain i of a batch follows, bit for bit in the FP64 form,
+ * the run of dnlg_create + dnlg_solve on the same inputs. */
+typedef struct dnlg_batch dnlg_batch;
+int dnlg_batch_create(int n_domains, const dnlg_mesh *meshes, const dnlg_material *materials, const dnlg_bc *bcs,
+                      const dnlg_solver_params *solver, dnlg_batch **out);
+int dnlg_batch_destroy(dnlg_batch *b);
+/* runs every domain to up_time (or its end_time); at most max_steps increments
+ * per domain (DNLG_E_STATE if a domain needs more).  wall_us[i] (optional) gets
+ * domain i's device wall time from its first to its last instruction,
+ * kernel_ms (optional) the launch's duration. */
+int dnlg_batch_solve(dnlg_batch *b, double up_time, long long max_steps, double *wall_us, double *kernel_ms);
+/* time state of domain i, the 9 values of dnlg_get_time */
+int dnlg_batch_get_time(dnlg_batch *b, int i, double *out9);
+/* a field of domain i: DNLG_F_COORDS/V/A/DU/D ([nn][3]), MASS ([nn]), P ([ne]),
+ * EVP ([ne][4] in the FP64 form, [4][ne] in the mixed one), S ([9][ne*4] FP64,
+ * [6][4][ne] mixed); n must equal the size */
+int dnlg_batch_get_field(dnlg_batch *b, int i, int field, double *out, long long n);
+const char *dnlg_batch_last_error(dnlg_batch *b);
+/* the parameter draw of the batched Taylor workload: for domain i, in this
+ * order, nr ~ U{5..20}, v0 ~ U(150, 300) m/s, sigma_y0 ~ U(300, 500) MPa,
+ * hardening modulus ~ U(50, 300) MPa, from std::mt19937_64(seed); an integer
+ * draw g gives nr = 5 + g % 16 and a real draw U(a, b) = a + (b - a) (g >> 11) 2^-53 */
+int dnlg_batch_taylor_params(int n, unsigned long long seed, int *nr, double *v0, double *sy0, double *hmod);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@ from ._lib import DnlgError, lib  # noqa: F401
 from .domain import (Domain, ExplicitSolverCH, IsoHardElastoplastic, IsoHardJohnsonCook,  # noqa: F401
                      Material)
 from . import mesh  # noqa: F401
+from .batch import Batch  # noqa: F401
 
 __all__ = ["Domain", "ExplicitSolverCH", "IsoHardElastoplastic", "IsoHardJohnsonCook", "Material", "mesh",
-           "DnlgError", "lib"]
+           "Batch", "DnlgError", "lib"]

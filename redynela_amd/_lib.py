@@ -24,6 +24,8 @@ EXPORTS = [
     "dnlg_begin", "dnlg_step", "dnlg_solve", "dnlg_step_async", "dnlg_synchronize", "dnlg_get_field",
     "dnlg_set_field", "dnlg_field_size", "dnlg_get_time", "dnlg_get_stream", "dnlg_enable_kernel_timing",
     "dnlg_kernel_stats", "dnlg_last_error", "dnlg_mesh_axi_bar", "dnlg_mesh_ogrid_sizes", "dnlg_mesh_ogrid_bar",
+    "dnlg_batch_create", "dnlg_batch_destroy", "dnlg_batch_solve", "dnlg_batch_get_time", "dnlg_batch_get_field",
+    "dnlg_batch_last_error", "dnlg_batch_taylor_params",
 ]
 
 
@@ -89,6 +91,13 @@ def lib():
             "dnlg_mesh_axi_bar": (i, [i, i, d, d, d, vp, vp, vp, vp]),
             "dnlg_mesh_ogrid_sizes": (i, [i, i, i, C.POINTER(ll), C.POINTER(ll)]),
             "dnlg_mesh_ogrid_bar": (i, [i, i, i, d, d, d, d, vp, vp, vp, vp]),
+            "dnlg_batch_create": (i, [i, vp, vp, vp, C.POINTER(SolverParams), C.POINTER(vp)]),
+            "dnlg_batch_destroy": (i, [vp]),
+            "dnlg_batch_solve": (i, [vp, d, ll, vp, C.POINTER(d)]),
+            "dnlg_batch_get_time": (i, [vp, i, vp]),
+            "dnlg_batch_get_field": (i, [vp, i, i, vp, ll]),
+            "dnlg_batch_last_error": (C.c_char_p, [vp]),
+            "dnlg_batch_taylor_params": (i, [i, C.c_ulonglong, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -20,9 +20,12 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-Xclang", "-t
           "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 UNITS = [
     ("dnlg_f64.hip", ["-ffp-contract=off"]),
-    ("dnlg_f32.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+    ("dnlg_f32.hip", ["-ffp-contract=fast-honor-pragmas", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+    ("dnlg_batch.hip", ["-ffp-contract=off"]),
+    ("dnlg_batch_f32.hip", ["-ffp-contract=fast-honor-pragmas", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("dnlg_api.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("dnlg_mesh.cpp", ["-ffp-contract=off", "-x", "hip"]),
+    ("dnlg_batch_api.cpp", ["-ffp-contract=off", "-x", "hip"]),
 ]
 
 

@@ -17,327 +17,9 @@
 //                          (FE/Solver.C:220-267, FE/ExplicitSolverCH.C:434-468)
 //   k_mass_elem / k_gather_scalar   Domain/Element::computeMassMatrix
 //                          (FE/Domain.C:948-1005, FE/Element.C:411-454)
-#include "dnlg_math.hpp"
-#include "dnlg_internal.hpp"
+#include "dnlg_elem_f64.hpp"
 
 namespace dnl {
-
-// signs of the reference element's node / Gauss point number k:
-// (-,-,-),(+,-,-),(+,+,-),(-,+,-),(-,-,+),(+,-,+),(+,+,+),(-,+,+)
-__device__ __forceinline__ int sg0(int k) { int q = k & 3; return (q == 1 || q == 2) ? 1 : -1; }
-__device__ __forceinline__ int sg1(int k) { return ((k & 3) >= 2) ? 1 : -1; }
-__device__ __forceinline__ int sg2(int k) { return (k >= 4) ? 1 : -1; }
-
-// hex8 dN0(ip, node, d): sign of node along d times (2+r3)/12, 1/12 or
-// (2-r3)/12 by how many of the two other directions the node and the Gauss
-// point share a sign (FE/ElHex8n3D.C:81-116 tables)
-__device__ __forceinline__ void hex_dn0(const ElemArgs &a, int ip, int k, double *g)
-{
-  const int a0 = sg0(ip) == sg0(k), a1 = sg1(ip) == sg1(k), a2 = sg2(ip) == sg2(k);
-  const int o0 = a1 + a2, o1 = a0 + a2, o2 = a0 + a1;
-  const double m0 = o0 == 2 ? a.c2 : (o0 == 1 ? a.c1 : a.c0);
-  const double m1 = o1 == 2 ? a.c2 : (o1 == 1 ? a.c1 : a.c0);
-  const double m2 = o2 == 2 ? a.c2 : (o2 == 1 ? a.c1 : a.c0);
-  g[0] = sg0(k) > 0 ? m0 : -m0;
-  g[1] = sg1(k) > 0 ? m1 : -m1;
-  g[2] = sg2(k) > 0 ? m2 : -m2;
-}
-// quad4 dN0(ip, node, d) = sign(node,d) * (3 +- r3)/12 (FE/ElQua4nAx.C:51-73)
-__device__ __forceinline__ void quad_dn0(const ElemArgs &a, int ip, int k, double *g)
-{
-  const double m0 = (sg1(ip) == sg1(k)) ? a.c1 : a.c0;
-  const double m1 = (sg0(ip) == sg0(k)) ? a.c1 : a.c0;
-  g[0] = sg0(k) > 0 ? m0 : -m0;
-  g[1] = sg1(k) > 0 ? m1 : -m1;
-}
-// quad4 N(ip, node): (2+r3)/6, 1/6, (2-r3)/6 by shared signs
-__device__ __forceinline__ double quad_n(const ElemArgs &a, int ip, int k)
-{
-  const int ag = (sg0(ip) == sg0(k)) + (sg1(ip) == sg1(k));
-  return ag == 2 ? a.n2 : (ag == 1 ? a.n1 : a.n0);
-}
-
-__device__ __forceinline__ void raise_status(StepState *st, int code, long long e)
-{
-  if (atomicCAS(&st->status, 0, code) == 0) st->err_elem = (int)e;
-}
-
-// J2 radial return and elastic predictor, FE/Element.C:248-348
-__device__ __forceinline__ bool stress_update(const ElemArgs &a, const T9 &dEps, T9 &S, double &evp, T9 *epl)
-{
-  const double trm = (dEps.v[0] + dEps.v[4] + dEps.v[8]) / 3.0;
-  T9 dev = dEps;
-  dev.v[0] = dEps.v[0] - trm;
-  dev.v[4] = dEps.v[4] - trm;
-  dev.v[8] = dEps.v[8] - trm;
-  const double g2 = 2.0 * a.G;
-#pragma unroll
-  for (int k = 0; k < 9; k++) S.v[k] += g2 * dev.v[k];
-  if (!a.yield) return true;
-  const double Sigv0 = a.A + a.B * det_pow(evp + kSqrt23 * 0.0, a.n);
-  const double Strial = kSqrt32 * sqrt(sumsq(S));
-  if (!(Strial > Sigv0)) return true;
-  double h = a.n * a.B * det_pow(evp + kSqrt23 * 0.0, a.nm1);
-  double gamma = (sqrt(sumsq(S)) - kSqrt23 * Sigv0) / (2.0 * a.G * (1.0 + (h) / (3.0 * a.G)));
-  int iteration = 0;
-  while (true)
-  {
-    const double ev = evp + kSqrt23 * gamma;
-    const double Sigv = a.A + a.B * det_pow(ev, a.n);
-    h = a.n * a.B * det_pow(ev, a.nm1);
-    const double dfun = kSqrt32 * 2.0 * a.G + kSqrt23 * h;
-    const double fun = dfun * gamma - (Strial - Sigv + h * kSqrt23 * gamma);
-    bool run = dabs(fun / Sigv) > kRRPrec;
-    if (run)
-    {
-      double dgamma = -fun / dfun;
-      if (dgamma < 0 && dabs(dgamma) > gamma) dgamma = -gamma * 0.9;
-      gamma += dgamma;
-    }
-    iteration++;
-    if (iteration > kRRMaxIt) return false;
-    if (!run) break;
-  }
-  const double nrm = sqrt(sumsq(S));
-  T9 n1;
-#pragma unroll
-  for (int k = 0; k < 9; k++) n1.v[k] = S.v[k] / nrm;
-  evp += kSqrt23 * gamma;
-  const double c = 2.0 * a.G * gamma;
-#pragma unroll
-  for (int k = 0; k < 9; k++) S.v[k] -= c * n1.v[k];
-  if (epl)
-#pragma unroll
-    for (int k = 0; k < 9; k++) epl->v[k] += gamma * n1.v[k];
-  return true;
-}
-
-// Johnson-Cook yield stress with rate and thermal factors (law 2, an
-// extension: FE/IsoHardJohnsonCook.h:51-68 documents the law, the reference
-// evaluates only A + B evp^n).  Newton on the consistency condition
-// Strial - sqrt(3/2) 2G gamma = sigma_y(evp + sqrt(2/3) gamma, epsdot, T),
-// same stop as FE/Element.C:304-331, kept inside a sign bracket (bisection
-// when a step leaves it: the rate factor has a kink at epsdot = epsdot0);
-// adiabatic heating with the Taylor-Quinney fraction after convergence.
-__device__ __forceinline__ double jc_theta(const ElemArgs &a, double T)
-{
-  if (a.chi_rho_cp == 0.0) return 1.0;
-  if (T < a.T0) return 1.0;
-  if (T >= a.Tm) return 0.0;
-  return 1.0 - det_pow((T - a.T0) / (a.Tm - a.T0), a.m);
-}
-
-__device__ __forceinline__ bool stress_update_jc(const ElemArgs &a, const T9 &dEps, T9 &S, double &evp, double &T,
-                                                 double dt, T9 *epl)
-{
-  const double trm = (dEps.v[0] + dEps.v[4] + dEps.v[8]) / 3.0;
-  T9 dev = dEps;
-  dev.v[0] = dEps.v[0] - trm;
-  dev.v[4] = dEps.v[4] - trm;
-  dev.v[8] = dEps.v[8] - trm;
-  const double g2 = 2.0 * a.G;
-#pragma unroll
-  for (int k = 0; k < 9; k++) S.v[k] += g2 * dev.v[k];
-  const double theta = jc_theta(a, T);
-  const double sy0 = (a.A + a.B * det_pow(evp, a.n)) * theta;
-  const double nrm = sqrt(sumsq(S));
-  const double Strial = kSqrt32 * nrm;
-  if (!(Strial > sy0)) return true;
-  const double h0 = a.n * a.B * det_pow(evp, a.nm1) * theta;
-  double gamma = (nrm - kSqrt23 * sy0) / (2.0 * a.G * (1.0 + h0 / (3.0 * a.G)));
-  double lo = 0.0, hi = Strial / (kSqrt32 * 2.0 * a.G);
-  if (!(gamma > lo && gamma < hi)) gamma = 0.5 * (lo + hi);
-  double sy = sy0;
-  int it = 0;
-  while (true)
-  {
-    const double ev = evp + kSqrt23 * gamma;
-    const double base = a.A + a.B * det_pow(ev, a.n);
-    const double dbase = a.n * a.B * det_pow(ev, a.nm1);
-    double rf = 1.0, drf = 0.0;
-    if (a.C != 0.0 && gamma > 0.0)
-    {
-      const double epsd = kSqrt23 * gamma / dt;
-      if (epsd >= a.epsd0)
-      {
-        rf = 1.0 + a.C * det_log(epsd / a.epsd0);
-        drf = a.C / gamma;
-      }
-    }
-    sy = base * rf * theta;
-    const double dsy = (kSqrt23 * dbase * rf + base * drf) * theta;
-    const double res = Strial - kSqrt32 * 2.0 * a.G * gamma - sy;
-    it++;
-    if (dabs(res / sy) <= kRRPrec) break;
-    if (it > kRRMaxIt) return false;
-    if (res > 0.0) lo = gamma; else hi = gamma;
-    const double gn = gamma + res / (kSqrt32 * 2.0 * a.G + dsy);
-    gamma = (gn > lo && gn < hi) ? gn : 0.5 * (lo + hi);
-  }
-  T9 n1;
-#pragma unroll
-  for (int k = 0; k < 9; k++) n1.v[k] = S.v[k] / nrm;
-  const double dep = kSqrt23 * gamma;
-  evp += dep;
-  const double c = 2.0 * a.G * gamma;
-#pragma unroll
-  for (int k = 0; k < 9; k++) S.v[k] -= c * n1.v[k];
-  if (epl)
-#pragma unroll
-    for (int k = 0; k < 9; k++) epl->v[k] += gamma * n1.v[k];
-  T += a.chi_rho_cp * sy * dep;
-  return true;
-}
-
-// ---------------------------------------------------------------------------
-// hex8 element kernel: lane = 8*element + integration point
-// ---------------------------------------------------------------------------
-// One integration point of one element per lane.  tid indexes the LDS
-// exchange buffer xs[19][W]; lanes 8k..8k+7 of a wave hold one element.
-template <bool SPEED_BC, bool TRACK, int XW>
-__device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_raw, int tid, double (*xs)[XW])
-{
-  StepState *st = a.st;
-  const int wl = tid & 63;
-  const int pt = wl & 7, base = wl & ~7, lbase = tid & ~7;
-  const bool valid = e_raw < a.ne;
-  const long long e = valid ? e_raw : a.ne - 1;
-  const long long nipt = a.ne * 8, ip = e * 8 + pt;
-  const int *cn = a.conn + e * 8;
-  int nd[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) nd[k] = cn[k];
-
-  // Jacobian at this Gauss point, FE/Element3D.C:75-89
-  T9 J = {{0, 0, 0, 0, 0, 0, 0, 0, 0}};
-#pragma unroll
-  for (int k = 0; k < 8; k++)
-  {
-    const double *xk = a.x + 3 * (long long)nd[k];
-    const double x0 = xk[0], x1 = xk[1], x2 = xk[2];
-    double g[3];
-    hex_dn0(a, pt, k, g);
-    J.v[0] += g[0] * x0; J.v[1] += g[1] * x0; J.v[2] += g[2] * x0;
-    J.v[3] += g[0] * x1; J.v[4] += g[1] * x1; J.v[5] += g[2] * x1;
-    J.v[6] += g[0] * x2; J.v[7] += g[1] * x2; J.v[8] += g[2] * x2;
-  }
-  const double detJ = (J.v[0] * J.v[4] * J.v[8]) + (J.v[3] * J.v[7] * J.v[2]) + (J.v[6] * J.v[1] * J.v[5]) -
-                      (J.v[6] * J.v[4] * J.v[2]) - (J.v[0] * J.v[7] * J.v[5]) - (J.v[3] * J.v[1] * J.v[8]);
-  T9 inv;
-  if (!inv3(J, inv) && valid) raise_status(st, ST_DETJ, e);
-
-  // F = I + sum_k dN(k,:) (x) du_k, FE/Element3D.C:182-205
-  T9 F = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
-#pragma unroll
-  for (int k = 0; k < 8; k++)
-  {
-    const double *uk = a.du + 3 * (long long)nd[k];
-    double u0 = uk[0], u1 = uk[1], u2 = uk[2];
-    if (SPEED_BC)
-    {
-      const unsigned char m = a.bcs[nd[k]];
-      if (m)
-      {
-        const double *sp = a.bcs_v + 3 * (long long)nd[k];
-        if (m & 1) u0 = sp[0] * st->dt;
-        if (m & 2) u1 = sp[1] * st->dt;
-        if (m & 4) u2 = sp[2] * st->dt;
-      }
-    }
-    double g[3], d[3];
-    hex_dn0(a, pt, k, g);
-    dgemm_row<3>(g, inv.v, d);
-    F.v[0] += d[0] * u0; F.v[1] += d[1] * u0; F.v[2] += d[2] * u0;
-    F.v[3] += d[0] * u1; F.v[4] += d[1] * u1; F.v[5] += d[2] * u1;
-    F.v[6] += d[0] * u2; F.v[7] += d[1] * u2; F.v[8] += d[2] * u2;
-  }
-
-  T9 dEps, R;
-  if (!polar_lnU(F, dEps, R) && valid) raise_status(st, ST_POLAR, e);
-
-  T9 S;
-#pragma unroll
-  for (int c = 0; c < 9; c++) S.v[c] = a.S[c * nipt + ip];
-  double evp = a.evp[ip];
-  T9 eps, epl;
-  if (TRACK)
-  {
-#pragma unroll
-    for (int c = 0; c < 9; c++) { eps.v[c] = a.eps[c * nipt + ip] + dEps.v[c]; epl.v[c] = a.epsplas[c * nipt + ip]; }
-  }
-  const double tr = dEps.v[0] + dEps.v[4] + dEps.v[8];
-  double Tip = 0.0;
-  bool rr_ok;
-  if (a.law == 2)
-  {
-    Tip = a.T[ip];
-    rr_ok = stress_update_jc(a, dEps, S, evp, Tip, st->dt, TRACK ? &epl : nullptr);
-  }
-  else
-    rr_ok = stress_update(a, dEps, S, evp, TRACK ? &epl : nullptr);
-  if (!rr_ok && valid) raise_status(st, ST_RR, e);
-
-  // pressure averaged over the element's points in point order, FE/Element.C:353-355
-  double pressure = 0.0;
-#pragma unroll
-  for (int q = 0; q < 8; q++) pressure += __shfl(tr, base + q, 64);
-  pressure /= 8;
-  const double p = a.p[e] + a.K * pressure;
-
-  // RFI and Sig = S + p I, FE/Element.C:362-401
-  S = mul(R, mul_bt(S, R));
-  if (TRACK) { eps = mul(R, mul_bt(eps, R)); epl = mul(R, mul_bt(epl, R)); }
-  T9 Sig;
-#pragma unroll
-  for (int k = 0; k < 9; k++) Sig.v[k] = S.v[k] + p * ((k % 4 == 0) ? 1.0 : 0.0);
-  const double WxdJ = 1.0 * detJ;
-
-  if (valid)
-  {
-#pragma unroll
-    for (int c = 0; c < 9; c++) a.S[c * nipt + ip] = S.v[c];
-    a.evp[ip] = evp;
-    if (a.law == 2) a.T[ip] = Tip;
-    if (pt == 0) a.p[e] = p;
-    if (TRACK)
-#pragma unroll
-      for (int c = 0; c < 9; c++) { a.eps[c * nipt + ip] = eps.v[c]; a.epsplas[c * nipt + ip] = epl.v[c]; }
-  }
-
-  // exchange J^-1, Sig and w detJ of the 8 points of the element
-#pragma unroll
-  for (int c = 0; c < 9; c++) { xs[c][tid] = inv.v[c]; xs[9 + c][tid] = Sig.v[c]; }
-  xs[18][tid] = WxdJ;
-  __syncthreads();
-
-  // nodal force of local node L = pt, accumulated point by point as
-  // Element::computeInternalForces does, FE/Element.C:178-202
-  const int L = pt;
-  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-#pragma unroll
-  for (int q = 0; q < 8; q++)
-  {
-    const int src = lbase + q;
-    double iq[9], sq[9];
-#pragma unroll
-    for (int c = 0; c < 9; c++) { iq[c] = xs[c][src]; sq[c] = xs[9 + c][src]; }
-    const double W = xs[18][src];
-    double g[3], d[3];
-    hex_dn0(a, q, L, g);
-    dgemm_row<3>(g, iq, d);
-#pragma unroll
-    for (int j = 0; j < 3; j++) f0 -= d[j] * sq[3 * j + 0] * W;
-#pragma unroll
-    for (int j = 0; j < 3; j++) f1 -= d[j] * sq[3 * j + 1] * W;
-#pragma unroll
-    for (int j = 0; j < 3; j++) f2 -= d[j] * sq[3 * j + 2] * W;
-  }
-  if (valid)
-  {
-    double *fo = a.fe + (long long)a.fpos[e * 8 + L] * 3;
-    fo[0] = f0; fo[1] = f1; fo[2] = f2;
-  }
-}
 
 template <bool SPEED_BC, bool TRACK>
 __global__ __launch_bounds__(64) void k_elem_hex8(ElemArgs a)
@@ -345,150 +27,7 @@ __global__ __launch_bounds__(64) void k_elem_hex8(ElemArgs a)
   if (a.check_running && !a.st->running) return;
   __shared__ double xs[19][64];
   const long long e_raw = ((long long)blockIdx.x * 64 + threadIdx.x) >> 3;
-  elem_hex8_body<SPEED_BC, TRACK, 64>(a, e_raw, threadIdx.x, xs);
-}
-
-// ---------------------------------------------------------------------------
-// axisymmetric quad4 element kernel: lane = 4*element + integration point
-// ---------------------------------------------------------------------------
-template <bool SPEED_BC, bool TRACK, int XW>
-__device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e_raw, int tid, double (*xs)[XW])
-{
-  StepState *st = a.st;
-  const int wl = tid & 63;
-  const int pt = wl & 3, base = wl & ~3, lbase = tid & ~3;
-  const bool valid = e_raw < a.ne;
-  const long long e = valid ? e_raw : a.ne - 1;
-  const long long nipt = a.ne * 4, ip = e * 4 + pt;
-  const int *cn = a.conn + e * 4;
-  int nd[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) nd[k] = cn[k];
-
-  // FE/Element2D.C:84-115
-  double J0 = 0, J1 = 0, J2 = 0, J3 = 0, Rr = 0.;
-#pragma unroll
-  for (int k = 0; k < 4; k++)
-  {
-    const double *xk = a.x + 3 * (long long)nd[k];
-    const double x0 = xk[0], x1 = xk[1];
-    double g[2];
-    quad_dn0(a, pt, k, g);
-    J0 += g[0] * x0; J1 += g[1] * x0;
-    J2 += g[0] * x1; J3 += g[1] * x1;
-  }
-  // getRadius_atIntPoint, FE/Element2D.C:233-246
-#pragma unroll
-  for (int k = 0; k < 4; k++) Rr += quad_n(a, pt, k) * a.x[3 * (long long)nd[k]];
-  const double detJ = (J0 * J3) - (J1 * J2);
-  if (detJ < 0. && valid) raise_status(st, ST_DETJ, e);
-  const double inv[4] = {J3 / detJ, -J1 / detJ, -J2 / detJ, J0 / detJ};
-
-  // ElementAx::getF_atIntPoint, FE/ElementAx.C:110-136
-  T9 F = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
-  double vr = 0.;
-#pragma unroll
-  for (int k = 0; k < 4; k++)
-  {
-    const double *uk = a.du + 3 * (long long)nd[k];
-    double u0 = uk[0], u1 = uk[1];
-    if (SPEED_BC)
-    {
-      const unsigned char m = a.bcs[nd[k]];
-      if (m)
-      {
-        const double *sp = a.bcs_v + 3 * (long long)nd[k];
-        if (m & 1) u0 = sp[0] * st->dt;
-        if (m & 2) u1 = sp[1] * st->dt;
-      }
-    }
-    double g[2], d[2];
-    quad_dn0(a, pt, k, g);
-    dgemm_row<2>(g, inv, d);
-    F.v[0] += d[0] * u0; F.v[1] += d[1] * u0;
-    F.v[3] += d[0] * u1; F.v[4] += d[1] * u1;
-    vr += quad_n(a, pt, k) * u0;
-  }
-  F.v[8] += vr / Rr;
-
-  T9 dEps, R;
-  if (!polar_lnU(F, dEps, R) && valid) raise_status(st, ST_POLAR, e);
-  T9 S;
-#pragma unroll
-  for (int c = 0; c < 9; c++) S.v[c] = a.S[c * nipt + ip];
-  double evp = a.evp[ip];
-  T9 eps, epl;
-  if (TRACK)
-  {
-#pragma unroll
-    for (int c = 0; c < 9; c++) { eps.v[c] = a.eps[c * nipt + ip] + dEps.v[c]; epl.v[c] = a.epsplas[c * nipt + ip]; }
-  }
-  const double tr = dEps.v[0] + dEps.v[4] + dEps.v[8];
-  double Tip = 0.0;
-  bool rr_ok;
-  if (a.law == 2)
-  {
-    Tip = a.T[ip];
-    rr_ok = stress_update_jc(a, dEps, S, evp, Tip, st->dt, TRACK ? &epl : nullptr);
-  }
-  else
-    rr_ok = stress_update(a, dEps, S, evp, TRACK ? &epl : nullptr);
-  if (!rr_ok && valid) raise_status(st, ST_RR, e);
-  double pressure = 0.0;
-#pragma unroll
-  for (int q = 0; q < 4; q++) pressure += __shfl(tr, base + q, 64);
-  pressure /= 4;
-  const double p = a.p[e] + a.K * pressure;
-  S = mul(R, mul_bt(S, R));
-  if (TRACK) { eps = mul(R, mul_bt(eps, R)); epl = mul(R, mul_bt(epl, R)); }
-  T9 Sig;
-#pragma unroll
-  for (int k = 0; k < 9; k++) Sig.v[k] = S.v[k] + p * ((k % 4 == 0) ? 1.0 : 0.0);
-  double WxdJ = 1.0 * detJ;
-  WxdJ *= kDeuxPI * Rr;
-
-  if (valid)
-  {
-#pragma unroll
-    for (int c = 0; c < 9; c++) a.S[c * nipt + ip] = S.v[c];
-    a.evp[ip] = evp;
-    if (a.law == 2) a.T[ip] = Tip;
-    if (pt == 0) a.p[e] = p;
-    if (TRACK)
-#pragma unroll
-      for (int c = 0; c < 9; c++) { a.eps[c * nipt + ip] = eps.v[c]; a.epsplas[c * nipt + ip] = epl.v[c]; }
-  }
-
-#pragma unroll
-  for (int c = 0; c < 4; c++) xs[c][tid] = inv[c];
-  xs[4][tid] = Sig.v[0]; xs[5][tid] = Sig.v[1]; xs[6][tid] = Sig.v[3]; xs[7][tid] = Sig.v[4];
-  xs[8][tid] = Sig.v[8]; xs[9][tid] = WxdJ; xs[10][tid] = Rr;
-  __syncthreads();
-
-  // FE/Element.C:178-212: in-plane terms, then the hoop term of this point
-  const int L = pt;
-  double f0 = 0.0, f1 = 0.0;
-#pragma unroll
-  for (int q = 0; q < 4; q++)
-  {
-    const int src = lbase + q;
-    const double iq[4] = {xs[0][src], xs[1][src], xs[2][src], xs[3][src]};
-    const double s00 = xs[4][src], s01 = xs[5][src], s10 = xs[6][src], s11 = xs[7][src], s22 = xs[8][src];
-    const double W = xs[9][src], Rq = xs[10][src];
-    double g[2], d[2];
-    quad_dn0(a, q, L, g);
-    dgemm_row<2>(g, iq, d);
-    f0 -= d[0] * s00 * W;
-    f0 -= d[1] * s10 * W;
-    f1 -= d[0] * s01 * W;
-    f1 -= d[1] * s11 * W;
-    f0 -= (quad_n(a, q, L) * s22 / Rq) * W;
-  }
-  if (valid)
-  {
-    double *fo = a.fe + (long long)a.fpos[e * 4 + L] * 2;
-    fo[0] = f0; fo[1] = f1;
-  }
+  elem_hex8_body<SPEED_BC, TRACK, 64>(a, e_raw, a.ne, threadIdx.x, xs);
 }
 
 template <bool SPEED_BC, bool TRACK>
@@ -497,7 +36,7 @@ __global__ __launch_bounds__(64) void k_elem_quad4ax(ElemArgs a)
   if (a.check_running && !a.st->running) return;
   __shared__ double xs[11][64];
   const long long e_raw = ((long long)blockIdx.x * 64 + threadIdx.x) >> 2;
-  elem_quad4ax_body<SPEED_BC, TRACK, 64>(a, e_raw, threadIdx.x, xs);
+  elem_quad4ax_body<SPEED_BC, TRACK, 64>(a, e_raw, a.ne, threadIdx.x, xs);
 }
 
 // ---------------------------------------------------------------------------
@@ -603,19 +142,6 @@ __global__ __launch_bounds__(256) void k_gather_fint(const unsigned char *deg, c
 // fused nodal step: predictor, BCs, Fint gather, corrector, BCs, grid update
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void apply_constant(unsigned char b, unsigned char s, const double *sp, double dt,
-                                               double *du, double *v, double *a, double *ac)
-{
-  // BoundarySpeed::applyConstant (constant speed), FE/Boundary.C:478-500
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-    if (s & (1 << c)) { a[c] = 0.; ac[c] = 0.; v[c] = sp[c]; du[c] = v[c] * dt; }
-  // BoundaryRestrain::applyConstant, FE/Boundary.C:636-666
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-    if (b & (1 << c)) { du[c] = 0.; v[c] = 0.; a[c] = 0.; ac[c] = 0.; }
-}
-
 // [nn][3] AoS arrays are moved through LDS so that every global load and
 // store of a block covers whole contiguous lines (3 x 256 doubles per field)
 __device__ __forceinline__ void stage_in(const double *g, double *l, long long i0, long long nn)
@@ -650,67 +176,10 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
   double ke = 0.0;
   if (i < n.nn)
   {
-    const double dt = st->dt, beta = st->beta, gam = st->gamma, am = st->alphaM;
-    const double s1 = dt * (0.5 - beta), s2 = dt * (1 - gam), s3 = (dt * gam), s4 = ((dt * dt) * beta);
-    double ac[3], vc[3], dun[3], vn[3], an[3];
+    double ac[3], vc[3], dun[3], vn[3], an[3], m;
 #pragma unroll
     for (int c = 0; c < 3; c++) { ac[c] = sa[t3 + c]; vc[c] = sv[t3 + c]; }
-    // ExplicitSolverCH::computePredictions, FE/ExplicitSolverCH.C:408-424
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-    {
-      double u = ac[c];
-      u *= s1;
-      u += vc[c];
-      u *= dt;
-      dun[c] = u;
-      double w = ac[c];
-      w *= s2;
-      w += vc[c];
-      vn[c] = w;
-      an[c] = 0.0;
-    }
-    const unsigned char b = n.bc[i];
-    const unsigned char s = HAS_BCS ? n.bcs[i] : 0;
-    const double *sp = HAS_BCS ? n.bcs_v + 3 * i : nullptr;
-    if (b | s) apply_constant(b, s, sp, dt, dun, vn, an, ac);
-    // Fint assembly in element order, then acc = Fint / M, FE/ExplicitSolverCH.C:506-519
-    double f[DIM];
-#pragma unroll
-    for (int j = 0; j < DIM; j++) f[j] = 0.0;
-    const int dg = n.deg[i];
-    for (int k = 0; k < dg; k++)
-    {
-      const long long q = ((long long)k * n.nn + i) * DIM;
-      if (FE32)
-      {
-#pragma unroll
-        for (int j = 0; j < DIM; j++) f[j] += (double)n.fe32[q + j];
-      }
-      else
-      {
-#pragma unroll
-        for (int j = 0; j < DIM; j++) f[j] += n.fe[q + j];
-      }
-    }
-    if (n.fint)
-#pragma unroll
-      for (int j = 0; j < DIM; j++) n.fint[i * DIM + j] = f[j];
-    const double m = n.mass[i];
-#pragma unroll
-    for (int j = 0; j < DIM; j++) an[j] = f[j] / m;
-    // FE/ExplicitSolverCH.C:522-541
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-    {
-      double aa = an[c];
-      aa -= am * ac[c];
-      aa /= (1.0 - am);
-      an[c] = aa;
-      vn[c] += s3 * aa;
-      dun[c] += s4 * aa;
-    }
-    if (b | s) apply_constant(b, s, sp, dt, dun, vn, an, ac);
+    nodal_core<DIM, HAS_BCS, FE32>(n, i, st, ac, vc, an, vn, dun, m);
 #pragma unroll
     for (int c = 0; c < 3; c++)
     {
@@ -745,44 +214,6 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
 // ---------------------------------------------------------------------------
 // Courant critical time per element, min-reduced per block
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double hex_length(const ElemArgs &a, const int *nd)
-{
-  // build-defined hex8 length: V / A_max (V = sum of the 8 detJ, weights 1)
-  double V = 0.0;
-  for (int pt = 0; pt < 8; pt++)
-  {
-    T9 J = {{0, 0, 0, 0, 0, 0, 0, 0, 0}};
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-    {
-      const double *xk = a.x + 3 * (long long)nd[k];
-      double g[3];
-      hex_dn0(a, pt, k, g);
-      J.v[0] += g[0] * xk[0]; J.v[1] += g[1] * xk[0]; J.v[2] += g[2] * xk[0];
-      J.v[3] += g[0] * xk[1]; J.v[4] += g[1] * xk[1]; J.v[5] += g[2] * xk[1];
-      J.v[6] += g[0] * xk[2]; J.v[7] += g[1] * xk[2]; J.v[8] += g[2] * xk[2];
-    }
-    V += (J.v[0] * J.v[4] * J.v[8]) + (J.v[3] * J.v[7] * J.v[2]) + (J.v[6] * J.v[1] * J.v[5]) -
-         (J.v[6] * J.v[4] * J.v[2]) - (J.v[0] * J.v[7] * J.v[5]) - (J.v[3] * J.v[1] * J.v[8]);
-  }
-  const int faces[6][4] = {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {7, 4, 5, 6}, {2, 1, 0, 3}};
-  double Amax = 0.0;
-#pragma unroll
-  for (int f = 0; f < 6; f++)
-  {
-    const double *xa = a.x + 3 * (long long)nd[faces[f][0]], *xb = a.x + 3 * (long long)nd[faces[f][1]],
-                 *xc = a.x + 3 * (long long)nd[faces[f][2]], *xd = a.x + 3 * (long long)nd[faces[f][3]];
-    const double d1[3] = {xc[0] - xa[0], xc[1] - xa[1], xc[2] - xa[2]};
-    const double d2[3] = {xd[0] - xb[0], xd[1] - xb[1], xd[2] - xb[2]};
-    const double c0 = d1[1] * d2[2] - d1[2] * d2[1];
-    const double c1 = d1[2] * d2[0] - d1[0] * d2[2];
-    const double c2 = d1[0] * d2[1] - d1[1] * d2[0];
-    const double area = 0.5 * sqrt(c0 * c0 + c1 * c1 + c2 * c2);
-    if (area > Amax) Amax = area;
-  }
-  return V / Amax;
-}
-
 template <int DIM>
 __global__ __launch_bounds__(256) void k_courant(ElemArgs a, double *part)
 {
@@ -800,15 +231,7 @@ __global__ __launch_bounds__(256) void k_courant(ElemArgs a, double *part)
     }
     else
     {
-      // ElQua4nAx::getLength, FE/ElQua4nAx.C:131-143
-      const int *cn = a.conn + e * 4;
-      const double *x0 = a.x + 3 * (long long)cn[0], *x1 = a.x + 3 * (long long)cn[1],
-                   *x2 = a.x + 3 * (long long)cn[2], *x3 = a.x + 3 * (long long)cn[3];
-      const double x31 = x2[0] - x0[0];
-      const double x24 = x1[0] - x3[0];
-      const double y42 = x3[1] - x1[1];
-      const double y31 = x2[1] - x0[1];
-      const double len = ((x31 * y42 + x24 * y31) / sqrt(2. * ((x24 * x24) + (y42 * y42) + (x31 * x31) + (y31 * y31))));
+      const double len = quad_length(a.x, a.conn + e * 4);
       t = len / (a.st->cs);
     }
   }
@@ -864,11 +287,8 @@ __global__ __launch_bounds__(256) void k_dt(StepState *st, const double *part, i
     go = 1;
     if (pending)
     {
-      st->t += st->dt;
       st->pending = 0;
-      const double lim = st->endTime < st->upTime ? st->endTime : st->upTime;
-      st->running = (st->t >= 0.0) && (st->t < lim);
-      go = st->running;
+      go = end_step(st);
       do_ts = go;
     }
     if (mode == 2) do_ts = 1;
@@ -890,28 +310,8 @@ __global__ __launch_bounds__(256) void k_dt(StepState *st, const double *part, i
     crit = red[0];
   }
   if (threadIdx.x != 0) return;
-  if (do_ts)
-  {
-    if (rec)
-    {
-      const double fmax = 2.0 / crit;
-      st->dt = st->safety * st->omegaS / fmax;
-      st->lastTsDate = st->tsDate;
-      st->lastTimeStep = st->computedTimeStep;
-      st->tsDate = st->t;
-      st->computedTimeStep = st->dt;
-    }
-    else if (st->tsDate != st->lastTsDate)
-    {
-      const double dx = (st->t - st->lastTsDate) / (st->tsDate - st->lastTsDate);
-      st->dt = ((dx) * (st->computedTimeStep) + (1.0 - (dx)) * (st->lastTimeStep));
-    }
-  }
-  if (mode == 0)
-  {
-    if (st->t + st->dt > st->upTime) st->dt = st->upTime - st->t;
-    st->increment++;
-  }
+  if (do_ts) compute_time_step(st, rec, crit);
+  if (mode == 0) init_step(st);
 }
 
 // ---------------------------------------------------------------------------

@@ -7,7 +7,7 @@
 namespace dnl {
 
 // device status codes (mirror include/dnlg.h)
-enum : int { ST_OK = 0, ST_POLAR = 2, ST_RR = 3, ST_DETJ = 4 };
+enum : int { ST_OK = 0, ST_POLAR = 2, ST_RR = 3, ST_DETJ = 4, ST_BUDGET = 6 };
 
 // Time-integration scalars kept on the device so that a step is launch-only.
 // Mirrors Solver / ExplicitSolverCH / Domain scalar state:
@@ -93,6 +93,26 @@ struct NodalArgs
   double *ke_part;  // optional per-block kinetic energy
   StepState *st;
   long long nn;
+};
+
+// one domain of a batch (packed into one device mesh)
+struct BarDesc
+{
+  long long node0, nn, elem0, ne;
+  double K, G, A, B, n, nm1;
+  int law, pad;
+};
+
+struct BatchArgs
+{
+  const BarDesc *bars;
+  const int *order;          // block b runs bar order[b] (largest first)
+  StepState *states;         // one per bar
+  ElemArgs ea;               // packed arrays, FP64 form
+  ElemArgs32 ea32;           // packed arrays, mixed form
+  NodalArgs na;              // packed arrays
+  unsigned long long *t0, *t1;  // wall clock at the start / end of each bar
+  long long max_steps;
 };
 
 }  // namespace dnl
