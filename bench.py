@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched-bars leg (FP64 and mixed)")
+    ap.add_argument("--batch-engine", default="auto", choices=["auto", "persistent", "stepped"])
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this build runs one domain on one GPU; --gpus %d is not supported "
@@ -52,6 +54,10 @@ def main():
     import redynela_amd as R
     from redynela_amd import _lib as L
 
+    if args.batch_only:
+        print(json.dumps({"batched": {form: batched_bars(form, engine=args.batch_engine)
+                                      for form in ("fp64", "mixed")}}), flush=True)
+        return
     mesh = R.mesh.ogrid_c1(args.layers)
     ne, nn = mesh["conn"].shape[0], mesh["x"].shape[0]
     mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
@@ -93,7 +99,7 @@ def main():
         fp32 = mixed_form(mesh, args, value)
     batched = None
     if not args.no_batch:
-        batched = {form: batched_bars(form) for form in ("fp64", "mixed")}
+        batched = {form: batched_bars(form, engine=args.batch_engine) for form in ("fp64", "mixed")}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args)
@@ -154,7 +160,7 @@ def mixed_form(mesh, args, fp64_value):
             "note": "same mesh, steps and warm-up; FP32 kinematics / stress / B^T sigma, FP64 nodal state and time"}
 
 
-def batched_bars(form, n=512, t_end=40e-6):
+def batched_bars(form, n=512, t_end=40e-6, engine="auto"):
     """The batched workload: 512 seeded Taylor bars (nr ~ U{5..20}, nz = 5 nr,
     125..2,000 quads, drawn speed and hardening) advanced to 40 us by one
     persistent launch; bars/s from the launch's HIP events, per-bar device
@@ -166,20 +172,27 @@ def batched_bars(form, n=512, t_end=40e-6):
     sol.setDissipation(1.0)
     sol.setPrecision(form)
     warm = R.Batch(meshes[:8], mats[:8], sol)  # loads the code object
+    warm.set_engine(engine)
     warm.solve(t_end)
     warm.close()
     b = R.Batch(meshes, mats, sol)
+    b.set_engine(engine)
     t0 = time.perf_counter()
     wall_us, ms = b.solve(t_end)
     host_s = time.perf_counter() - t0
     steps = np.array([b.time(i)["increment"] for i in range(n)])
     ne = np.array([m["conn"].shape[0] for m in meshes])
     b.close()
-    return {"bars": n, "bars_per_s": n / (ms * 1e-3), "launch_ms": ms, "host_solve_s": host_s,
+    nr, v0, sy0, hm = R.batch.taylor_params(n)
+    slow = [{"bar": int(i), "nr": int(nr[i]), "v0": float(v0[i]), "sigma_y0": float(sy0[i]), "H": float(hm[i]),
+             "increments": int(steps[i]), "wall_us": float(wall_us[i])} for i in np.argsort(-steps)[:5]]
+    return {"bars": n, "engine": engine, "bars_per_s": n / (ms * 1e-3), "solve_ms": ms, "host_solve_s": host_s,
             "p50_us_per_bar": float(np.percentile(wall_us, 50)), "p99_us_per_bar": float(np.percentile(wall_us, 99)),
             "max_us_per_bar": float(wall_us.max()), "element_steps": int((ne * steps).sum()),
             "element_steps_per_s": float((ne * steps).sum() / (ms * 1e-3)),
-            "increments_min_max": [int(steps.min()), int(steps.max())], "elements_total": int(ne.sum())}
+            "increments_min_max": [int(steps.min()), int(steps.max())],
+            "increments_p50_p99": [float(np.percentile(steps, 50)), float(np.percentile(steps, 99))],
+            "elements_total": int(ne.sum()), "most_increments": slow}
 
 
 def profiled_traffic():

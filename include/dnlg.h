@@ -197,10 +197,20 @@ typedef struct dnlg_batch dnlg_batch;
 int dnlg_batch_create(int n_domains, const dnlg_mesh *meshes, const dnlg_material *materials, const dnlg_bc *bcs,
                       const dnlg_solver_params *solver, dnlg_batch **out);
 int dnlg_batch_destroy(dnlg_batch *b);
+/* engines: PERSISTENT = one workgroup carries one domain through all of its
+ * increments in a single launch; STEPPED = one increment of every running
+ * domain per group of launches (element pass, per-domain time step, nodal
+ * phase), replayed from a captured hipGraph; AUTO (default) = STEPPED.
+ * Results are bit-identical across engines. */
+#define DNLG_BATCH_AUTO 0
+#define DNLG_BATCH_PERSISTENT 1
+#define DNLG_BATCH_STEPPED 2
+int dnlg_batch_set_engine(dnlg_batch *b, int engine);
 /* runs every domain to up_time (or its end_time); at most max_steps increments
  * per domain (DNLG_E_STATE if a domain needs more).  wall_us[i] (optional) gets
- * domain i's device wall time from its first to its last instruction,
- * kernel_ms (optional) the launch's duration. */
+ * domain i's device wall time from the start of its solve to the end of its
+ * last increment, kernel_ms (optional) the duration of the whole solve on the
+ * device stream. */
 int dnlg_batch_solve(dnlg_batch *b, double up_time, long long max_steps, double *wall_us, double *kernel_ms);
 /* time state of domain i, the 9 values of dnlg_get_time */
 int dnlg_batch_get_time(dnlg_batch *b, int i, double *out9);

@@ -25,8 +25,9 @@ EXPORTS = [
     "dnlg_set_field", "dnlg_field_size", "dnlg_get_time", "dnlg_get_stream", "dnlg_enable_kernel_timing",
     "dnlg_kernel_stats", "dnlg_last_error", "dnlg_mesh_axi_bar", "dnlg_mesh_ogrid_sizes", "dnlg_mesh_ogrid_bar",
     "dnlg_batch_create", "dnlg_batch_destroy", "dnlg_batch_solve", "dnlg_batch_get_time", "dnlg_batch_get_field",
-    "dnlg_batch_last_error", "dnlg_batch_taylor_params",
+    "dnlg_batch_last_error", "dnlg_batch_taylor_params", "dnlg_batch_set_engine",
 ]
+BATCH_AUTO, BATCH_PERSISTENT, BATCH_STEPPED = 0, 1, 2
 
 
 class Mesh(C.Structure):
@@ -98,6 +99,7 @@ def lib():
             "dnlg_batch_get_field": (i, [vp, i, i, vp, ll]),
             "dnlg_batch_last_error": (C.c_char_p, [vp]),
             "dnlg_batch_taylor_params": (i, [i, C.c_ulonglong, vp, vp, vp, vp]),
+            "dnlg_batch_set_engine": (i, [vp, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -80,6 +80,12 @@ class Batch:
         if rc != 0:
             raise DnlgError("%s: %s" % (_lib.ERRORS.get(rc, rc), _lib.lib().dnlg_batch_last_error(self.h).decode()))
 
+    def set_engine(self, engine):
+        """'auto', 'persistent' (one workgroup per domain, one launch) or
+        'stepped' (one increment of every domain per launch group)."""
+        e = {"auto": _lib.BATCH_AUTO, "persistent": _lib.BATCH_PERSISTENT, "stepped": _lib.BATCH_STEPPED}[engine]
+        self._check(_lib.lib().dnlg_batch_set_engine(self.h, e))
+
     def solve(self, upTime, max_steps=200000):
         """Every domain to upTime, at most max_steps increments each (a domain
         that needs more raises); returns (per-domain device wall time in us,

@@ -1,11 +1,108 @@
-// Persistent batched solver, FP64 form (compiled with -ffp-contract=off).
+// Batched solver, FP64 form (compiled with -ffp-contract=off): the persistent
+// kernel's FP64 instance, and the stepped engine's element pass, per-bar time
+// step and nodal phase (the last two serve both forms).
 #include "dnlg_batch.hpp"
 
 namespace dnl {
 
+// ---------------------------------------------------------------------------
+// Stepped engine: every bar advances by one increment per group of three
+// launches (element pass over all running bars, per-bar time step, nodal
+// phase over all active bars), each bar with its own time step.  The order of
+// operations inside a bar is that of ExplicitSolverCH::solve, so a bar stays
+// bit-identical to its single-domain run.  Block tables map a block to
+// (bar, first element / node); blocks never straddle two bars.
+// ---------------------------------------------------------------------------
+
+// per-bar scalar phase.  preamble: computeTimeStep(True) after the preamble's
+// element pass.  Otherwise, for each running bar: computeTimeStep (Courant
+// minimum from this increment's element pass when increment % freq == 0),
+// initStep, and -- ahead of the nodal phase, which reads neither t nor the
+// running flag -- the endStep time update and loop condition.
+__global__ __launch_bounds__(64) void k_bstep_dt(BatchArgs B, int nb, int preamble)
+{
+  const int bar = blockIdx.x * 64 + threadIdx.x;
+  if (bar >= nb) return;
+  StepState *st = B.states + bar;
+  const double crit = __longlong_as_double((long long)B.crit[bar]);
+  B.crit[bar] = 0x7FF0000000000000ull;  // +inf
+  if (preamble)
+  {
+    st->active = 0;
+    if (st->running)
+    {
+      B.t0[bar] = wall_clock64();
+      compute_time_step(st, 1, crit);
+    }
+    return;
+  }
+  if (!st->running || st->status)
+  {
+    st->active = 0;
+    return;
+  }
+  const int rec = (st->increment % st->freq) == 0;
+  compute_time_step(st, rec, crit);
+  init_step(st);
+  st->active = 1;
+  if (!end_step(st)) B.t1[bar] = wall_clock64();
+}
+
+template <bool FE32>
+__global__ __launch_bounds__(64) void k_bstep_nodal(BatchArgs B, const int2 *blk)
+{
+  const int2 bb = blk[blockIdx.x];
+  const StepState *st = B.states + bb.x;
+  if (!st->active) return;
+  const BarDesc &d = B.bars[bb.x];
+  const long long i = (long long)bb.y + threadIdx.x;
+  if (i >= d.node0 + d.nn) return;
+  bar_node<FE32>(B.na, i, st);
+}
+
+// FP64 element pass: 16 elements (64 points) per block
+__global__ __launch_bounds__(64) void k_bstep_elem(BatchArgs B, const int2 *blk, int preamble)
+{
+  __shared__ double xs[11][64];
+  const int2 bb = blk[blockIdx.x];
+  StepState *st = B.states + bb.x;
+  if (!st->running || st->status) return;
+  const BarDesc d = B.bars[bb.x];
+  ElemArgs a = B.ea;
+  a.st = st;
+  a.K = d.K; a.G = d.G; a.A = d.A; a.B = d.B; a.n = d.n; a.nm1 = d.nm1;
+  a.yield = d.law ? 1 : 0;
+  a.law = d.law;
+  const long long e_end = d.elem0 + d.ne, e_raw = (long long)bb.y + (threadIdx.x >> 2);
+  elem_quad4ax_body<false, false, 64>(a, e_raw, e_end, threadIdx.x, xs);
+  if (preamble || (st->increment % st->freq) == 0)
+  {
+    double l = __builtin_inf();
+    if ((threadIdx.x & 3) == 0 && e_raw < e_end) l = quad_length(a.x, a.conn + e_raw * 4) / st->cs;
+    courant_contrib(l, B.crit + bb.x);
+  }
+}
+
+
 void launch_batch_f64(const BatchArgs &B, int nbars, hipStream_t s)
 {
   hipLaunchKernelGGL((k_batch_quad4<ElemPassF64, false>), dim3(nbars), dim3(256), 0, s, B);
+}
+
+void launch_bstep_elem_f64(const BatchArgs &B, const int2 *blk, int nblk, int preamble, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_bstep_elem, dim3(nblk), dim3(64), 0, s, B, blk, preamble);
+}
+
+void launch_bstep_dt(const BatchArgs &B, int nb, int preamble, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_bstep_dt, dim3((nb + 63) / 64), dim3(64), 0, s, B, nb, preamble);
+}
+
+void launch_bstep_nodal(const BatchArgs &B, const int2 *blk, int nblk, bool fe32, hipStream_t s)
+{
+  if (fe32) hipLaunchKernelGGL(k_bstep_nodal<true>, dim3(nblk), dim3(64), 0, s, B, blk);
+  else hipLaunchKernelGGL(k_bstep_nodal<false>, dim3(nblk), dim3(64), 0, s, B, blk);
 }
 
 }  // namespace dnl

@@ -28,27 +28,42 @@ __device__ __forceinline__ double bar_courant(const ElemArgs &a, long long e0, l
   return c;
 }
 
-// one pass of the time loop's nodal phase over the bar's nodes
+// the time loop's nodal phase for node i: predictor, element-order force
+// gather, corrector, BCs and the grid update (lagMotion::updateNode)
+template <bool FE32>
+__device__ __forceinline__ void bar_node(const NodalArgs &n, long long i, const StepState *st)
+{
+#pragma clang fp contract(off)
+  double ac[3], vc[3], an[3], vn[3], dun[3], m;
+#pragma unroll
+  for (int c = 0; c < 3; c++) { ac[c] = n.a[3 * i + c]; vc[c] = n.v[3 * i + c]; }
+  nodal_core<2, false, FE32>(n, i, st, ac, vc, an, vn, dun, m);
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+  {
+    n.a[3 * i + c] = an[c];
+    n.v[3 * i + c] = vn[c];
+    n.du[3 * i + c] = dun[c];
+    n.d[3 * i + c] = n.d[3 * i + c] + dun[c];
+    n.x[3 * i + c] = n.x[3 * i + c] + dun[c];
+  }
+}
+
 template <bool FE32>
 __device__ __forceinline__ void bar_nodal(const NodalArgs &n, long long i0, long long i1, const StepState *st)
 {
-#pragma clang fp contract(off)
-  for (long long i = i0 + threadIdx.x; i < i1; i += 256)
-  {
-    double ac[3], vc[3], an[3], vn[3], dun[3], m;
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) bar_node<FE32>(n, i, st);
+}
+
+// element a's contribution to its bar's Courant minimum (one lane per element,
+// lanes of other points pass +inf): a wave minimum, then one atomic per wave.
+// The minimum of positive doubles is exact in any order, and positive doubles
+// order like their bit patterns.
+__device__ __forceinline__ void courant_contrib(double l, unsigned long long *crit)
+{
 #pragma unroll
-    for (int c = 0; c < 3; c++) { ac[c] = n.a[3 * i + c]; vc[c] = n.v[3 * i + c]; }
-    nodal_core<2, false, FE32>(n, i, st, ac, vc, an, vn, dun, m);
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-    {
-      n.a[3 * i + c] = an[c];
-      n.v[3 * i + c] = vn[c];
-      n.du[3 * i + c] = dun[c];
-      n.d[3 * i + c] = n.d[3 * i + c] + dun[c];
-      n.x[3 * i + c] = n.x[3 * i + c] + dun[c];  // lagMotion::updateNode
-    }
-  }
+  for (int o = 32; o > 0; o >>= 1) l = fmin(l, __shfl_xor(l, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMin(crit, (unsigned long long)__double_as_longlong(l));
 }
 
 // FP64 element pass over one bar: 64 elements (4 points each) per sweep of 256 lanes

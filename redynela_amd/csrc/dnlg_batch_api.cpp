@@ -16,6 +16,10 @@
 namespace dnl {
 void launch_batch_f64(const BatchArgs &B, int nbars, hipStream_t s);
 void launch_batch_f32(const BatchArgs &B, int nbars, hipStream_t s);
+void launch_bstep_elem_f64(const BatchArgs &B, const int2 *blk, int nblk, int preamble, hipStream_t s);
+void launch_bstep_elem_f32(const BatchArgs &B, const int2 *blk, int nblk, int preamble, hipStream_t s);
+void launch_bstep_dt(const BatchArgs &B, int nb, int preamble, hipStream_t s);
+void launch_bstep_nodal(const BatchArgs &B, const int2 *blk, int nblk, bool fe32, hipStream_t s);
 void launch_mass_f64(int dim, const ElemArgs &a, double rho, const double *hn, double *me, hipStream_t s);
 void launch_gather_scalar(const unsigned char *deg, const double *src, double *dst, long long nn, hipStream_t s);
 }  // namespace dnl
@@ -38,6 +42,16 @@ struct dnlg_batch
   StepState *st = nullptr;
   unsigned long long *t0 = nullptr, *t1 = nullptr;
   BatchArgs args{};
+  // stepped engine: block tables (bar, first element / node), largest bars first
+  // (device tables hold the blocks of the running bars only; they are
+  // compacted on the host as bars finish, and the graph is re-captured)
+  int2 *blk_e = nullptr, *blk_n = nullptr;
+  int nblk_e = 0, nblk_n = 0;
+  std::vector<std::vector<int2>> bar_be, bar_bn;  // every block of each bar
+  unsigned long long *crit = nullptr;
+  hipGraphExec_t graph = nullptr;  // kGraphSteps increments over the current tables
+  int graph_nblk_e = -1, graph_nblk_n = -1;
+  int engine = DNLG_BATCH_AUTO;
   bool mass_done = false;
   double rho = 0;
   std::string err;
@@ -62,10 +76,14 @@ static hipError_t balloc(T **p, size_t n)
   return hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
 }
 
+static constexpr int kGraphSteps = 8;
+
 static void bfree(dnlg_batch *b)
 {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
   void *ptrs[] = {b->x, b->v, b->a, b->du, b->d, b->mass, b->me, b->fe, b->S, b->evp, b->p, b->fe32, b->S32,
-                  b->evp32, b->p32, b->conn, b->fpos, b->order_d, b->deg, b->bc, b->bars_d, b->st, b->t0, b->t1};
+                  b->evp32, b->p32, b->conn, b->fpos, b->order_d, b->deg, b->bc, b->bars_d, b->st, b->t0, b->t1,
+                  b->blk_e, b->blk_n, b->crit};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -276,6 +294,26 @@ extern "C" int dnlg_batch_create(int n, const dnlg_mesh *meshes, const dnlg_mate
   BCK(hipMemcpy(b->bars_d, b->bars.data(), sizeof(BarDesc) * n, hipMemcpyHostToDevice));
   BCK(hipMemcpy(b->order_d, b->order.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   BCK(hipMemcpy(b->st, st.data(), sizeof(StepState) * n, hipMemcpyHostToDevice));
+  {
+    // stepped engine block tables: FP64 16 elements (64 points) per block, FP32 64 elements, 64 nodes
+    const int per_e = b->form == DNLG_FORM_FP64 ? 16 : 64;
+    b->bar_be.resize(n);
+    b->bar_bn.resize(n);
+    long long tot_e = 0, tot_n = 0;
+    for (int i = 0; i < n; i++)
+    {
+      const BarDesc &d = b->bars[i];
+      for (long long e = 0; e < d.ne; e += per_e) b->bar_be[i].push_back(make_int2(i, (int)(d.elem0 + e)));
+      for (long long q = 0; q < d.nn; q += 64) b->bar_bn[i].push_back(make_int2(i, (int)(d.node0 + q)));
+      tot_e += (long long)b->bar_be[i].size();
+      tot_n += (long long)b->bar_bn[i].size();
+    }
+    std::vector<unsigned long long> inf(n, 0x7FF0000000000000ull);
+    BCK(balloc(&b->blk_e, tot_e));
+    BCK(balloc(&b->blk_n, tot_n));
+    BCK(balloc(&b->crit, n));
+    BCK(hipMemcpy(b->crit, inf.data(), sizeof(unsigned long long) * n, hipMemcpyHostToDevice));
+  }
 #undef BCK
   // kernel arguments over the packed arrays
   BatchArgs &A = b->args;
@@ -284,6 +322,7 @@ extern "C" int dnlg_batch_create(int n, const dnlg_mesh *meshes, const dnlg_mate
   A.states = b->st;
   A.t0 = b->t0;
   A.t1 = b->t1;
+  A.crit = b->crit;
   ElemArgs &ea = A.ea;
   ea.conn = b->conn;
   ea.x = b->x;
@@ -336,6 +375,90 @@ extern "C" int dnlg_batch_destroy(dnlg_batch *b)
   return DNLG_OK;
 }
 
+static void issue_stepped(dnlg_batch *b, int preamble)
+{
+  const bool f64 = b->form == DNLG_FORM_FP64;
+  if (f64) launch_bstep_elem_f64(b->args, b->blk_e, b->nblk_e, preamble, b->stream);
+  else launch_bstep_elem_f32(b->args, b->blk_e, b->nblk_e, preamble, b->stream);
+  launch_bstep_dt(b->args, b->nb, preamble, b->stream);
+  if (!preamble) launch_bstep_nodal(b->args, b->blk_n, b->nblk_n, !f64, b->stream);
+}
+
+// upload the block tables of the bars still running (largest bars first) and
+// re-capture the step graph when the tables shrank by a quarter or more
+static int compact_tables(dnlg_batch *b, const std::vector<StepState> &st, bool force)
+{
+  std::vector<int2> be, bn;
+  for (int k = 0; k < b->nb; k++)
+  {
+    const int i = b->order[k];
+    if (!st[i].running || st[i].status) continue;
+    be.insert(be.end(), b->bar_be[i].begin(), b->bar_be[i].end());
+    bn.insert(bn.end(), b->bar_bn[i].begin(), b->bar_bn[i].end());
+  }
+  if (!force && b->graph && 4 * (long long)be.size() > 3 * (long long)b->graph_nblk_e) return DNLG_OK;
+  b->nblk_e = (int)be.size();
+  b->nblk_n = (int)bn.size();
+  if (be.empty()) return DNLG_OK;
+  BHIP(hipMemcpyAsync(b->blk_e, be.data(), sizeof(int2) * be.size(), hipMemcpyHostToDevice, b->stream));
+  BHIP(hipMemcpyAsync(b->blk_n, bn.data(), sizeof(int2) * bn.size(), hipMemcpyHostToDevice, b->stream));
+  BHIP(hipStreamSynchronize(b->stream));  // the host vectors go out of scope
+  if (b->graph)
+  {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  hipGraph_t g;
+  BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < kGraphSteps; k++) issue_stepped(b, 0);
+  BHIP(hipStreamEndCapture(b->stream, &g));
+  const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  BHIP(e);
+  b->graph_nblk_e = b->nblk_e;
+  b->graph_nblk_n = b->nblk_n;
+  return DNLG_OK;
+}
+
+// the stepped engine: the preamble, then increments in chunks (hipGraph
+// replays of kGraphSteps increments) until no bar runs or max_steps are issued
+static int solve_stepped(dnlg_batch *b, long long max_steps, std::vector<StepState> &st, long long *issued)
+{
+  int rc = compact_tables(b, st, true);
+  if (rc) return rc;
+  *issued = 0;
+  if (b->nblk_e == 0) return DNLG_OK;
+  issue_stepped(b, 1);
+  BHIP(hipGetLastError());
+  long long done = 0, chunk = 32;
+  while (done < max_steps)
+  {
+    const long long n = std::min(chunk, max_steps - done);
+    long long k = 0;
+    for (; k + kGraphSteps <= n; k += kGraphSteps) BHIP(hipGraphLaunch(b->graph, b->stream));
+    for (; k < n; k++) issue_stepped(b, 0);
+    BHIP(hipGetLastError());
+    done += n;
+    BHIP(hipMemcpyAsync(st.data(), b->st, sizeof(StepState) * b->nb, hipMemcpyDeviceToHost, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    bool any = false;
+    for (const StepState &s : st) any = any || (s.running && !s.status);
+    if (!any) break;
+    rc = compact_tables(b, st, false);
+    if (rc) return rc;
+    chunk = std::min<long long>(chunk * 2, 512);
+  }
+  *issued = done;
+  return DNLG_OK;
+}
+
+extern "C" int dnlg_batch_set_engine(dnlg_batch *b, int engine)
+{
+  if (!b || engine < DNLG_BATCH_AUTO || engine > DNLG_BATCH_STEPPED) return DNLG_E_ARG;
+  b->engine = engine;
+  return DNLG_OK;
+}
+
 extern "C" int dnlg_batch_solve(dnlg_batch *b, double up_time, long long max_steps, double *wall_us, double *kernel_ms)
 {
   if (!b || max_steps <= 0) return DNLG_E_ARG;
@@ -358,13 +481,25 @@ extern "C" int dnlg_batch_solve(dnlg_batch *b, double up_time, long long max_ste
     b->mass_done = true;
   }
   b->args.max_steps = max_steps;
+  BHIP(hipMemsetAsync(b->t0, 0, sizeof(unsigned long long) * b->nb, b->stream));
+  BHIP(hipMemsetAsync(b->t1, 0, sizeof(unsigned long long) * b->nb, b->stream));
   hipEvent_t e0, e1;
   BHIP(hipEventCreate(&e0));
   BHIP(hipEventCreate(&e1));
   BHIP(hipEventRecord(e0, b->stream));
-  if (b->form == DNLG_FORM_FP64) launch_batch_f64(b->args, b->nb, b->stream);
-  else launch_batch_f32(b->args, b->nb, b->stream);
-  BHIP(hipGetLastError());
+  long long issued = 0;
+  const bool stepped = b->engine != DNLG_BATCH_PERSISTENT;
+  if (!stepped)
+  {
+    if (b->form == DNLG_FORM_FP64) launch_batch_f64(b->args, b->nb, b->stream);
+    else launch_batch_f32(b->args, b->nb, b->stream);
+    BHIP(hipGetLastError());
+  }
+  else
+  {
+    int rc = solve_stepped(b, max_steps, st, &issued);
+    if (rc) return rc;
+  }
   BHIP(hipEventRecord(e1, b->stream));
   BHIP(hipStreamSynchronize(b->stream));
   float ms = 0.f;
@@ -380,9 +515,12 @@ extern "C" int dnlg_batch_solve(dnlg_batch *b, double up_time, long long max_ste
     int dev = 0, khz = 0;
     BHIP(hipGetDevice(&dev));
     BHIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    for (int i = 0; i < b->nb; i++) wall_us[i] = khz > 0 ? (double)(t1[i] - t0[i]) * 1e3 / (double)khz : 0.0;
+    for (int i = 0; i < b->nb; i++) wall_us[i] = (khz > 0 && t1[i] > t0[i]) ? (double)(t1[i] - t0[i]) * 1e3 / (double)khz : 0.0;
   }
   BHIP(hipMemcpy(st.data(), b->st, sizeof(StepState) * b->nb, hipMemcpyDeviceToHost));
+  if (stepped)
+    for (int i = 0; i < b->nb; i++)
+      if (!st[i].status && st[i].running && issued >= max_steps) st[i].status = ST_BUDGET;
   for (int i = 0; i < b->nb; i++)
     if (st[i].status)
     {

@@ -22,4 +22,30 @@ void launch_batch_f32(const BatchArgs &B, int nbars, hipStream_t s)
   hipLaunchKernelGGL((k_batch_quad4<ElemPassF32, true>), dim3(nbars), dim3(256), 0, s, B);
 }
 
+// stepped engine, mixed form: one element per lane, 64 elements per block
+__global__ __launch_bounds__(64) void k_bstep_elem32(BatchArgs B, const int2 *blk, int preamble)
+{
+  const int2 bb = blk[blockIdx.x];
+  StepState *st = B.states + bb.x;
+  if (!st->running || st->status) return;
+  const BarDesc d = B.bars[bb.x];
+  ElemArgs32 a = B.ea32;
+  a.st = st;
+  a.K = (float)d.K; a.G = (float)d.G; a.A = (float)d.A; a.B = (float)d.B; a.n = (float)d.n;
+  a.law = d.law;
+  const long long e_end = d.elem0 + d.ne, e = (long long)bb.y + threadIdx.x;
+  if (e < e_end) elem_quad4ax_f32_body(a, e);
+  if (preamble || (st->increment % st->freq) == 0)
+  {
+    // Courant length in FP64 on the FP64 coordinates, as in the FP64 form
+    const double l = e < e_end ? quad_length(B.ea.x, B.ea.conn + e * 4) / st->cs : __builtin_inf();
+    courant_contrib(l, B.crit + bb.x);
+  }
+}
+
+void launch_bstep_elem_f32(const BatchArgs &B, const int2 *blk, int nblk, int preamble, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_bstep_elem32, dim3(nblk), dim3(64), 0, s, B, blk, preamble);
+}
+
 }  // namespace dnl

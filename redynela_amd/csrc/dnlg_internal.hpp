@@ -28,7 +28,7 @@ struct StepState
   int status;        // first device error
   int err_elem;
   int pending;       // the last issued increment still needs its endStep
-  int pad;
+  int active;        // batched stepped engine: this increment runs its nodal phase
 };
 
 struct ElemArgs
@@ -113,6 +113,7 @@ struct BatchArgs
   NodalArgs na;              // packed arrays
   unsigned long long *t0, *t1;  // wall clock at the start / end of each bar
   long long max_steps;
+  unsigned long long *crit;  // stepped engine: per-bar Courant minimum (bits of a positive double)
 };
 
 }  // namespace dnl

@@ -1,8 +1,9 @@
-"""GPU: the batched mode (one persistent launch, one workgroup per bar).
+"""GPU: the batched mode (many bars, each with its own time step).
 
-* FP64: sampled bars of the 512-bar workload, run together to 40 us, are
-  bit-identical to the oracle running each bar alone, and to the
-  single-domain GPU path (also across a split solve);
+* FP64: sampled bars of the 512-bar workload, run together to 40 us by the
+  stepped engine, are bit-identical to the oracle running each bar alone, and
+  to the single-domain GPU path (also across a split solve); the persistent
+  engine gives the same bits;
 * mixed form: every bar's final length and mushroom radius within 1e-4 of
   the FP64 batch, peak plastic strain within 2e-3;
 * an exhausted increment budget is an error; a solve with nothing to do
@@ -102,6 +103,27 @@ def test_batch_bar_equals_single_domain_gpu_run_split_solve(workload):
         assert np.array_equal(b.evp(1), d.evp)
     b.close()
     d.close()
+
+
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_persistent_engine_equals_stepped(workload, form):
+    import redynela_amd as R
+    from redynela_amd import _lib as L
+    meshes, mats = workload
+    idx = list(range(0, 512, 21))
+    say("case: %d bars, %s, persistent vs stepped engine, to 40 us" % (len(idx), form))
+    out = []
+    for eng in ("persistent", "stepped"):
+        b = R.Batch([meshes[i] for i in idx], [mats[i] for i in idx], solver(form))
+        b.set_engine(eng)
+        w, ms = b.solve(T_END)
+        say("  %s: %.2f ms" % (eng, ms))
+        out.append([(b.time(k)["increment"], b.coords(k), b.evp(k), b.field(k, L.F_S)) for k in range(len(idx))])
+        b.close()
+    for p, q in zip(*out):
+        assert p[0] == q[0]
+        for u, v in zip(p[1:], q[1:]):
+            assert np.array_equal(u, v)
 
 
 def test_batch_mixed_tracks_fp64(workload, fp64_batch):
