@@ -225,6 +225,32 @@ const char *dnlg_batch_last_error(dnlg_batch *b);
  * draw g gives nr = 5 + g % 16 and a real draw U(a, b) = a + (b - a) (g >> 11) 2^-53 */
 int dnlg_batch_taylor_params(int n, unsigned long long seed, int *nr, double *v0, double *sy0, double *hmod);
 
+/* ---- adjoint: gradient of a function of the final coordinates w.r.t. (A, B, n) ----
+ * No reference counterpart.  For an axisymmetric FP64 handle with hardening
+ * A + B evp^n (and no speed BC) that has not stepped yet:
+ * dnlg_adjoint_forward runs the solve() preamble and nsteps increments
+ * (the same bits as dnlg_begin + dnlg_step) and keeps the state after every
+ * ckpt_every-th increment in device memory; dnlg_adjoint_backward takes the
+ * adjoint xbar ([n_nodes][3], the third component ignored) of the final node
+ * coordinates and returns grad = (dL/dA, dL/dB, dL/dn).  The reverse sweep
+ * restores each checkpoint, recomputes its segment and reverses it: the
+ * Chung-Hulbert update, the restraints, the element-order assembly, the
+ * element kinematics (closed-form 2x2 polar decomposition), the radial return
+ * with its elastic/plastic switch, and the time step (Courant arg-min element,
+ * LINEARdbl extrapolation).  After it the handle holds the post-forward state. */
+typedef struct
+{
+  double forward_ms;      /* dnlg_adjoint_forward, device time incl. checkpoint copies  */
+  double backward_ms;     /* dnlg_adjoint_backward, device time                         */
+  double recompute_ms;    /* part of backward_ms spent recomputing segments             */
+  long long device_bytes; /* checkpoints, segment tape and adjoint state                */
+  long long checkpoint_bytes;
+  int n_checkpoints;
+  long long steps;
+} dnlg_adjoint_stats;
+int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_every);
+int dnlg_adjoint_backward(dnlg_handle *h, const double *xbar, double *grad, dnlg_adjoint_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -207,6 +207,11 @@ static void t2_rfi(const double *R, double *S)
   t2_mul(R, SRt, S);
 }
 
+/* the Jacobi stop |(F^T F)_pq| < 1e-9 of LA/Tensor2.C:34; test-only override
+ * (a smaller stop removes the shear dead zone below it from finite differences) */
+static double o_polar_prec = O_POLAR_PREC;
+void dnlo_set_polar_precision(double tol) { o_polar_prec = tol; }
+
 /* Tensor2::polarDecomposeLnU, LA/Tensor2.C:1154-1259.
  * F (row-major) -> U = ln(sqrt(F^T F)) and R = F (F^T F)^(-1/2). */
 int dnlo_polar_lnU(const double *F, double *U, double *R, int mathmode)
@@ -220,7 +225,7 @@ int dnlo_polar_lnU(const double *F, double *U, double *R, int mathmode)
     q = (O_ABS(FF[1]) > O_ABS(FF[2]) ? 1 : 2);
     p = (O_ABS(FF[q]) > O_ABS(FF[5]) ? 0 : 1);
     if (p == 1) q = 2;
-    if (O_ABS(FF[3 * p + q]) < O_POLAR_PREC)
+    if (O_ABS(FF[3 * p + q]) < o_polar_prec)
     {
       double lam[3] = {FF[0], FF[4], FF[8]};
       double Ru[9];
@@ -479,6 +484,7 @@ typedef struct
   long increment;
   double computedTimeStep, timeStepComputationDate, lastTimeStepComputationDate, lastTimeStep;
   int mathmode, mass_done, ke_on;
+  double rr_prec; /* radial-return stop, 1e-8 as the reference (FE/DefaultValues.h:29) */
   double ke;
   int status;
 } dnlo;
@@ -547,6 +553,7 @@ dnlo *dnlo_create(int dim, long nn, long ne, const double *x0, const int *conn, 
   m->rho_b = 0.8182; /* FE/DefaultValues.h:25 */
   m->safety = 0.9;
   m->freq = 10;
+  m->rr_prec = O_RR_PREC;
   m->endTime = 1e300;
   m->yield = 1;
   return m;
@@ -592,6 +599,9 @@ void dnlo_set_solver(dnlo *m, double rho_b, double safety, int freq, double endT
 }
 
 void dnlo_set_mathmode(dnlo *m, int mode) { m->mathmode = mode; }
+/* test-only: a tighter radial-return stop than the reference's 1e-8, to take
+ * the iteration-count jumps out of finite differences of the solve */
+void dnlo_set_rr_precision(dnlo *m, double tol) { m->rr_prec = tol; }
 void dnlo_set_ke(dnlo *m, int on) { m->ke_on = on; }
 void dnlo_set_threads(int n)
 {
@@ -718,7 +728,7 @@ static int o_stress_jc(dnlo *m, long ip, double *S, double G)
     double dsy = (O_SQRT23 * dbase * rf + base * drf) * theta;
     double res = Strial - O_SQRT32 * 2.0 * G * gamma - sy;
     it++;
-    if (O_ABS(res / sy) <= O_RR_PREC) break;
+    if (O_ABS(res / sy) <= m->rr_prec) break;
     if (it > O_RR_MAXIT) return O_E_RR;
     if (res > 0.0) lo = gamma; else hi = gamma;
     double gn = gamma + res / (O_SQRT32 * 2.0 * G + dsy);
@@ -806,7 +816,7 @@ static int o_compute_stress(dnlo *m, long e)
           h = m->n * m->B * o_pow(mm, ev, (m->n - 1.));
           dfun = O_SQRT32 * 2.0 * G + O_SQRT23 * h;
           fun = dfun * gamma - (Strial - Sigv + h * O_SQRT23 * gamma);
-          if (O_ABS(fun / Sigv) > O_RR_PREC)
+          if (O_ABS(fun / Sigv) > m->rr_prec)
           {
             dgamma = -fun / dfun;
             if (dgamma < 0 && O_ABS(dgamma) > gamma) dgamma = -gamma * 0.9;

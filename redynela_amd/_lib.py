@@ -26,6 +26,7 @@ EXPORTS = [
     "dnlg_kernel_stats", "dnlg_last_error", "dnlg_mesh_axi_bar", "dnlg_mesh_ogrid_sizes", "dnlg_mesh_ogrid_bar",
     "dnlg_batch_create", "dnlg_batch_destroy", "dnlg_batch_solve", "dnlg_batch_get_time", "dnlg_batch_get_field",
     "dnlg_batch_last_error", "dnlg_batch_taylor_params", "dnlg_batch_set_engine",
+    "dnlg_adjoint_forward", "dnlg_adjoint_backward",
 ]
 BATCH_AUTO, BATCH_PERSISTENT, BATCH_STEPPED = 0, 1, 2
 
@@ -52,6 +53,12 @@ class SolverParams(C.Structure):
 
 
 FORM_FP64, FORM_MIXED = 0, 1
+
+
+class AdjointStats(C.Structure):
+    _fields_ = [("forward_ms", C.c_double), ("backward_ms", C.c_double), ("recompute_ms", C.c_double),
+                ("device_bytes", C.c_longlong), ("checkpoint_bytes", C.c_longlong), ("n_checkpoints", C.c_int),
+                ("steps", C.c_longlong)]
 
 
 class DnlgError(RuntimeError):
@@ -100,6 +107,8 @@ def lib():
             "dnlg_batch_last_error": (C.c_char_p, [vp]),
             "dnlg_batch_taylor_params": (i, [i, C.c_ulonglong, vp, vp, vp, vp]),
             "dnlg_batch_set_engine": (i, [vp, i]),
+            "dnlg_adjoint_forward": (i, [vp, ll, i]),
+            "dnlg_adjoint_backward": (i, [vp, vp, vp, C.POINTER(AdjointStats)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -26,6 +26,8 @@ UNITS = [
     ("dnlg_api.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("dnlg_mesh.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("dnlg_batch_api.cpp", ["-ffp-contract=off", "-x", "hip"]),
+    ("dnlg_adjoint.hip", ["-ffp-contract=fast-honor-pragmas"]),
+    ("dnlg_adjoint_api.cpp", ["-ffp-contract=off", "-x", "hip"]),
 ]
 
 

@@ -10,8 +10,7 @@
 //               (endStep of the previous increment, computeTimeStep, initStep)
 //               element kernel, fused nodal kernel
 // and a final k_dt at every synchronisation point closes the last increment.
-#include "../../include/dnlg.h"
-#include "dnlg_internal.hpp"
+#include "dnlg_handle.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -31,67 +30,13 @@ void launch_courant_f64(int dim, const ElemArgs &a, double *part, hipStream_t s)
 void launch_dt(StepState *st, const double *part, int npart, int recompute, const double *ke_part, int nke, int mode,
                hipStream_t s);
 unsigned nodal_blocks(long long nn);
+void adj_free(AdjStore *s);
 unsigned courant_blocks(long long ne);
 }  // namespace dnl
 
 using namespace dnl;
 
-struct dnlg_handle
-{
-  int dim = 0, nen = 0, nip = 0;
-  long long nn = 0, ne = 0;
-  hipStream_t stream = nullptr;
-  // device buffers
-  double *x = nullptr, *v = nullptr, *a = nullptr, *du = nullptr, *d = nullptr, *mass = nullptr;
-  double *fint = nullptr, *fe = nullptr, *S = nullptr, *evp = nullptr, *p = nullptr;
-  double *eps = nullptr, *epsplas = nullptr, *me = nullptr, *part = nullptr, *ke_part = nullptr;
-  double *bcs_v = nullptr;
-  int *conn = nullptr, *fpos = nullptr;
-  unsigned char *deg = nullptr;
-  int maxdeg = 0;
-  std::vector<int> fpos_host;  // (element, local node) -> slot*nn + node
-  unsigned char *bc = nullptr, *bcs = nullptr;
-  StepState *st = nullptr;
-  bool has_bcs = false;
-  int form = DNLG_FORM_FP64;
-  double *T = nullptr;                       // FP64 point temperature (hardening 2)
-  float *S32 = nullptr, *evp32 = nullptr, *T32 = nullptr, *p32 = nullptr, *fe32 = nullptr;  // mixed form
-  ElemArgs32 ea32{};
-  // options
-  bool ke_on = false, store_fint = true, track_d = true, track_strain = false;
-  // host-side sequencing
-  bool mass_done = false, pending = false, begun = false;
-  long long inc = 0;
-  int freq = 10;
-  dnlg_material mat{};
-  double hn[4] = {0, 0, 0, 0};
-  ElemArgs ea{};
-  // timing
-  bool timing = false;
-  std::vector<hipEvent_t> ev_elem, ev_nodal;  // start/stop pairs
-  std::vector<hipEvent_t> ev_pool;
-  size_t ev_used = 0;
-  std::string err;
-};
-
 static thread_local std::string g_err;
-
-#define DNLG_HIP(h, call)                                                                        \
-  do                                                                                             \
-  {                                                                                              \
-    hipError_t _e = (call);                                                                      \
-    if (_e != hipSuccess)                                                                        \
-    {                                                                                            \
-      (h)->err = std::string("HIP error ") + hipGetErrorString(_e) + " at " #call;               \
-      return DNLG_E_HIP;                                                                         \
-    }                                                                                            \
-  } while (0)
-
-template <typename T>
-static hipError_t dalloc(T **p, size_t n)
-{
-  return hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
-}
 
 extern "C" int dnlg_abi_version(void) { return DNLG_ABI_VERSION; }
 
@@ -208,6 +153,8 @@ static NodalArgs nodal_args(dnlg_handle *h)
 
 static void free_all(dnlg_handle *h)
 {
+  adj_free(h->adj);
+  h->adj = nullptr;
   void *ptrs[] = {h->x, h->v, h->a, h->du, h->d, h->mass, h->fint, h->fe, h->S, h->evp, h->p, h->eps, h->epsplas,
                   h->me, h->part, h->ke_part, h->bcs_v, h->conn, h->fpos, h->deg, h->bc, h->bcs, h->st, h->T,
                   h->S32, h->evp32, h->T32, h->p32, h->fe32};

@@ -116,4 +116,34 @@ struct BatchArgs
   unsigned long long *crit;  // stepped engine: per-bar Courant minimum (bits of a positive double)
 };
 
+// adjoint of one increment (dnlg_adjoint.hip): scalar adjoints and kernel arguments
+struct AdjScalars
+{
+  double tb, dtb_carry, tsb, ltsb, Cb, Lb;  // adjoints of t, dt (carried), tsDate, lastTsDate, computed, last time step
+  double th[3];                             // accumulated dL/dA, dL/dB, dL/dn
+};
+
+struct AdjArgs
+{
+  ElemArgs ea;                      // shape constants, conn, fpos, ne (other fields unused)
+  const unsigned char *bc, *deg;
+  const double *mass;
+  long long nn, ne;
+  // forward tape: state after increment k (x, v, a, du: [nn][3]; S [9][4 ne]; evp [4 ne]; p [ne]) and a_{k+1}
+  const double *x, *v, *a, *du, *S, *evp, *p, *a1;
+  const StepState *st0, *st1;       // time state after increments k and k+1
+  // adjoint state, updated in place from increment k+1 to k
+  double *xb, *ub, *vb, *ab;        // [nn][2]
+  double *Sb, *evb, *pb;            // [9][4 ne], [4 ne], [ne]
+  double *fb;                       // [nn][2] adjoint of the assembled internal force
+  double *slot_x, *slot_u;          // [maxdeg][nn][2] per (element, node) contributions, element order
+  double *dt_part, *th_part;        // per-block partial sums
+  int ndt_part, nth_part;
+  AdjScalars *sc;
+  double K, G, A, B, n;
+  double alphaM, beta, gamma, safety, omegaS, cs;
+  int freq;
+  long long k;                      // this launch reverses increment k+1
+};
+
 }  // namespace dnl

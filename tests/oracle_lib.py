@@ -48,6 +48,8 @@ def lib():
         L.dnlo_set_material.argtypes = [C.c_void_p] + [C.c_double] * 6 + [C.c_int]
         L.dnlo_set_solver.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double]
         L.dnlo_set_mathmode.argtypes = [C.c_void_p, C.c_int]
+        L.dnlo_set_rr_precision.argtypes = [C.c_void_p, C.c_double]
+        L.dnlo_set_polar_precision.argtypes = [C.c_double]
         L.dnlo_set_johnson_cook.argtypes = [C.c_void_p] + [C.c_double] * 7
         L.dnlo_set_ke.argtypes = [C.c_void_p, C.c_int]
         L.dnlo_set_speed_bc.argtypes = [C.c_void_p, C.c_long, C.c_int, _dp]
@@ -105,6 +107,8 @@ class Model:
         L.dnlo_set_solver(self.h, solver.get("rho_b", 0.8182), solver.get("safety", 0.9),
                           int(solver.get("freq", 10)), solver.get("end_time", 1e300))
         L.dnlo_set_mathmode(self.h, mathmode)
+        if "rr_prec" in solver:  # test-only: tighter radial-return stop than the reference's 1e-8
+            L.dnlo_set_rr_precision(self.h, solver["rr_prec"])
         if "jc" in mat:  # Johnson-Cook rate / thermal extension
             j = mat["jc"]
             L.dnlo_set_johnson_cook(self.h, j["C"], j["epsd0"], j["m"], j["T0"], j["Tm"], j["cp"], j["tq"])

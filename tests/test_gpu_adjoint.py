@@ -1,0 +1,154 @@
+"""GPU: the hand-written adjoint (dnlg_adjoint_forward / _backward).
+
+* its forward is the production stepping, bit for bit;
+* its gradient equals the numpy restatement's adjoint (tests/adjoint_ref.py)
+  on a small bar, with uneven checkpoint segments;
+* on the parameter-identification workload (axi 20 x 100, 200 m/s, 4,000
+  increments, checkpoints every 100) the gradient of the final-shape misfit
+  w.r.t. (A, B, n) matches central differences of the oracle (the C
+  restatement of the reference solver): within 1e-5 at relative step 1e-6
+  once the solver's stops are tightened, and within its noise floor (2e-4)
+  with the reference's own stops;
+* the torch.autograd.Function returns the same gradient.
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import adjoint_ref as AR
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_final(mesh, theta, nsteps, smooth=False):
+    """Final (r, z) of the oracle run.  smooth: test-only tighter stops of the
+    radial return (1e-13 instead of 1e-8) and of the Jacobi polar iteration
+    (1e-15 instead of 1e-9), which take the reference's iteration-count jumps
+    and its shear dead zone |(F^T F)_01| < 1e-9 out of the finite differences."""
+    A, B, n = theta
+    sol = dict(rho_b=1.0)
+    if smooth:
+        sol["rr_prec"] = 1e-13
+    m = O.Model(mesh, dict(E=115e9, nu=0.3, rho=8930.0, A=A, B=B, n=n), sol, mathmode=1)
+    O.lib().dnlo_set_polar_precision(1e-15 if smooth else 10.0e-10)
+    try:
+        m.begin(1e300)
+        for _ in range(nsteps):
+            m.step()
+    finally:
+        O.lib().dnlo_set_polar_precision(10.0e-10)
+    return m.get(O.F_X).reshape(-1, 3)[:, :2]
+
+
+def central(mesh, th, i, h, nst, loss, smooth=False):
+    tp, tm = th.copy(), th.copy()
+    tp[i] *= 1 + h
+    tm[i] *= 1 - h
+    return (loss(oracle_final(mesh, tp, nst, smooth)) - loss(oracle_final(mesh, tm, nst, smooth))) / (2 * h * th[i])
+
+
+def test_adjoint_forward_is_the_production_stepping():
+    import redynela_amd as R
+    from redynela_amd.adjoint import HardeningAdjoint
+    say("case: axi 4x16, 150 increments, checkpoints every 40: adjoint forward == dnlg_step")
+    mesh = R.mesh.taylor_axi(4, 16, 3.2e-3, 5.2e-3, 200.0)
+    th = (90e6, 292e6, 0.31)
+    p = HardeningAdjoint(mesh, nsteps=150, ckpt_every=40)
+    x = p.forward(th)
+    s = R.ExplicitSolverCH()
+    s.setDissipation(1.0)
+    d = R.Domain(mesh, R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(*th)), s)
+    d.begin(1e300)
+    d.step(150)
+    assert np.array_equal(x, d.coords)
+    # the backward leaves the post-forward state in place
+    p.backward(np.ones((x.shape[0], 2)))
+    assert np.array_equal(p.dom.coords, d.coords)
+    assert p.dom.time()["increment"] == 150
+    p.close()
+    d.close()
+
+
+def test_adjoint_equals_numpy_restatement():
+    import redynela_amd as R
+    from redynela_amd.adjoint import HardeningAdjoint
+    say("case: axi 4x16, 230 increments, checkpoints every 60 (uneven), GPU vs numpy adjoint")
+    mesh = R.mesh.taylor_axi(4, 16, 3.2e-3, 5.2e-3, 200.0)
+    th0 = np.array([90e6, 292e6, 0.31])
+    nst = 230
+    m = AR.Model(mesh)
+    xt, _ = m.run(th0, nst)
+    th = th0 * np.array([1.12, 0.9, 1.1])
+    xr, gr = m.grad(th, nst, xt)
+    p = HardeningAdjoint(mesh, nsteps=nst, ckpt_every=60)
+    x = p.forward(th)[:, :2]
+    g = p.backward(2 * (x - xt))
+    say("  numpy", gr, "\n  gpu  ", g, "\n  rel  ", np.abs(g - gr) / np.abs(gr))
+    assert np.max(np.abs(x - xr)) < 1e-10 * np.max(np.abs(xr))  # Jacobi vs closed-form polar, round-off
+    assert np.all(np.abs(g - gr) <= 1e-7 * np.abs(gr))
+    p.close()
+
+
+def test_adjoint_c4_matches_oracle_central_differences():
+    from redynela_amd import adjoint as AD
+    mesh = AD.c4_mesh()
+    nst = AD.C4_STEPS
+    say("case: axi 20x100, 200 m/s, %d increments, checkpoints every %d; target (A,B,n) = (90 MPa, 292 MPa, 0.31)"
+        % (nst, AD.C4_CKPT))
+    xt = oracle_final(mesh, AD.C4_THETA, nst)
+    th = AD.c4_start()
+    p = AD.HardeningAdjoint(mesh, nsteps=nst, ckpt_every=AD.C4_CKPT)
+    t0 = time.perf_counter()
+    x = p.forward(th)[:, :2]
+    g = p.backward(2 * (x - xt))
+    wall = time.perf_counter() - t0
+    st = p.stats
+    say("  theta", th, "\n  adjoint grad", g)
+    say("  forward %.1f ms, backward %.1f ms (recompute %.1f ms), device memory %.1f MB (%d checkpoints, %.1f MB), "
+        "wall %.2f s" % (st["forward_ms"], st["backward_ms"], st["recompute_ms"], st["device_bytes"] / 1e6,
+                         st["n_checkpoints"], st["checkpoint_bytes"] / 1e6, wall))
+    xo = oracle_final(mesh, th, nst)
+    assert np.array_equal(x, xo)  # the forward is the oracle's, bit for bit
+    # central differences of the reference solver: its map has jumps of
+    # ~1e-9 (radial-return iteration count, Jacobi shear dead zone), so the
+    # difference quotient has a noise floor; reported, bounded loosely
+    loss = lambda xx: np.sum((xx - xt) ** 2)  # noqa: E731
+    for i in range(3):
+        row = [(h, central(mesh, th, i, h, nst, loss)) for h in (1e-4, 1e-5)]
+        rel = [abs(fd - g[i]) / abs(fd) for _, fd in row]
+        say("  reference FD, param %d: " % i + "  ".join("h=%.0e fd=%.9e rel=%.2e" % (h, fd, r)
+                                                        for (h, fd), r in zip(row, rel)))
+        assert min(rel) <= 2e-4
+    # central differences of the smoothed solver at relative step 1e-6: <= 1e-5
+    xts = oracle_final(mesh, AD.C4_THETA, nst, smooth=True)
+    loss_s = lambda xx: np.sum((xx - xts) ** 2)  # noqa: E731
+    for i in range(3):
+        fd = central(mesh, th, i, 1e-6, nst, loss_s, smooth=True)
+        rel = abs(fd - g[i]) / abs(fd)
+        say("  smoothed FD, param %d: h=1e-06 fd=%.9e rel=%.2e" % (i, fd, rel))
+        assert rel <= 1e-5
+    p.close()
+
+
+def test_torch_autograd_function():
+    import torch
+    import redynela_amd as R
+    from redynela_amd import adjoint as AD
+    say("case: torch.autograd through FinalShape, axi 4x16, 120 increments")
+    mesh = R.mesh.taylor_axi(4, 16, 3.2e-3, 5.2e-3, 200.0)
+    prob = AD.HardeningAdjoint(mesh, nsteps=120, ckpt_every=50)
+    xt = torch.from_numpy(prob.forward(AD.C4_THETA)[:, :2].copy())
+    theta = torch.tensor([100e6, 280e6, 0.3], dtype=torch.float64, requires_grad=True)
+    loss = ((AD.FinalShape.apply(theta, prob) - xt) ** 2).sum()
+    loss.backward()
+    g = prob.backward(2 * (prob.dom.coords[:, :2] - xt.numpy()))
+    assert torch.allclose(theta.grad, torch.from_numpy(g), rtol=1e-12, atol=0)
+    prob.close()
