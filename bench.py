@@ -45,6 +45,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-bars leg (FP64 and mixed)")
     ap.add_argument("--batch-engine", default="auto", choices=["auto", "persistent", "stepped"])
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="only the O-grid Johnson-Cook bar (4,000 elements per section) in both forms")
+    ap.add_argument("--precision-layers", type=int, default=1000)
+    ap.add_argument("--precision-tend", type=float, default=60e-6)
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this build runs one domain on one GPU; --gpus %d is not supported "
@@ -54,6 +59,10 @@ def main():
     import redynela_amd as R
     from redynela_amd import _lib as L
 
+    if args.precision_report:
+        print(json.dumps({"precision_report": precision_report(args.precision_layers, args.precision_tend)}),
+              flush=True)
+        return
     if args.batch_only:
         print(json.dumps({"batched": {form: batched_bars(form, engine=args.batch_engine)
                                       for form in ("fp64", "mixed")}}), flush=True)
@@ -100,6 +109,9 @@ def main():
     batched = None
     if not args.no_batch:
         batched = {form: batched_bars(form, engine=args.batch_engine) for form in ("fp64", "mixed")}
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args)
@@ -130,6 +142,7 @@ def main():
                           "nodal_kernel_avg_ms": avg_nd * 1e3},
         "fp32_form": fp32,
         "batched": batched,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -193,6 +206,68 @@ def batched_bars(form, n=512, t_end=40e-6, engine="auto"):
             "increments_min_max": [int(steps.min()), int(steps.max())],
             "increments_p50_p99": [float(np.percentile(steps, 50)), float(np.percentile(steps, 99))],
             "elements_total": int(ne.sum()), "most_increments": slow}
+
+
+def adjoint_leg():
+    """Gradient of the final-shape misfit w.r.t. (A, B, n) on the axisymmetric
+    20 x 100 bar, 4,000 increments, checkpoints every 100; time and memory of
+    the reverse sweep against one forward run."""
+    import numpy as np
+    from redynela_amd import adjoint as AD
+    mesh = AD.c4_mesh()
+    p = AD.HardeningAdjoint(mesh)
+    xt = p.forward(AD.C4_THETA)[:, :2].copy()
+    t0 = time.perf_counter()
+    x = p.forward(AD.c4_start())[:, :2]
+    g = p.backward(2 * (x - xt))
+    wall = time.perf_counter() - t0
+    st = p.stats
+    p.close()
+    return {"workload": "axi 20x100 quads, 200 m/s, 4000 increments, checkpoints every 100, FP64",
+            "grad": [float(v) for v in g], "forward_ms": st["forward_ms"], "backward_ms": st["backward_ms"],
+            "recompute_ms": st["recompute_ms"], "backward_over_forward": st["backward_ms"] / st["forward_ms"],
+            "device_MB": st["device_bytes"] / 1e6, "checkpoint_MB": st["checkpoint_bytes"] / 1e6,
+            "checkpoints": st["n_checkpoints"], "wall_s": wall}
+
+
+def precision_report(layers=1000, t_end=60e-6):
+    """The O-grid bar of 4,000 hexahedra per section (40x40 core + 4 rings of
+    40x15; 1,000 layers = 4.0M elements), 250 m/s, Johnson-Cook with rate and
+    adiabatic heating, to t_end in the FP64 and the mixed form: final length,
+    mushroom radius and peak plastic strain, and their relative differences."""
+    import numpy as np
+    import redynela_amd as R
+    mesh = R.mesh.ogrid_bar(40, 15, layers, 3.2e-3, 32.4e-3, 0.4, 250.0)
+    base = mesh["x"][:, 2] == 0.0
+    out = {"workload": "hex8 O-grid 4,000 el/section x %d layers (%d elements), 250 m/s, Johnson-Cook "
+                       "A=90MPa B=292MPa n=0.31 C=0.025 m=1.09 adiabatic, to %.0f us"
+                       % (layers, mesh["conn"].shape[0], t_end * 1e6)}
+    for form in ("fp64", "mixed"):
+        h = R.IsoHardJohnsonCook(90e6, 292e6, 0.31, C=0.025, m=1.09, depsp0=1.0, Tm=1356.0, T0=293.0)
+        h.enableRateAndThermal(383.0, 0.9)
+        sol = R.ExplicitSolverCH()
+        sol.setDissipation(1.0)
+        sol.setPrecision(form)
+        d = R.Domain(mesh, R.Material(115e9, 0.3, 8930.0, h), sol)
+        d.set_option(2, 0)
+        t0 = last = time.perf_counter()
+        d.begin(t_end)
+        while d.step(256):
+            if time.perf_counter() - last > 15.0:
+                last = time.perf_counter()
+                tt = d.time()
+                print("  %s: increment %d, t = %.3e s, dt = %.3e s" % (form, tt["increment"], tt["t"], tt["dt"]),
+                      file=sys.stderr, flush=True)
+        el = time.perf_counter() - t0
+        n = d.time()["increment"]
+        x = d.coords
+        out[form] = {"increments": n, "solve_s": el, "length": float(x[:, 2].max()),
+                     "radius": float(np.hypot(x[base, 0], x[base, 1]).max()), "max_evp": float(d.evp.max())}
+        d.close()
+    a, b = out["fp64"], out["mixed"]
+    out["rel_err"] = {k: abs(b[k] - a[k]) / abs(a[k]) for k in ("length", "radius", "max_evp")}
+    out["speedup_mixed_vs_fp64"] = a["solve_s"] / b["solve_s"]
+    return out
 
 
 def profiled_traffic():

@@ -63,18 +63,16 @@ __global__ __launch_bounds__(64) void k_bstep_nodal(BatchArgs B, const int2 *blk
 // FP64 element pass: 16 elements (64 points) per block
 __global__ __launch_bounds__(64) void k_bstep_elem(BatchArgs B, const int2 *blk, int preamble)
 {
-  __shared__ double xs[11][64];
+  __shared__ QuadLds<64> Ls;
   const int2 bb = blk[blockIdx.x];
   StepState *st = B.states + bb.x;
   if (!st->running || st->status) return;
-  const BarDesc d = B.bars[bb.x];
-  ElemArgs a = B.ea;
-  a.st = st;
-  a.K = d.K; a.G = d.G; a.A = d.A; a.B = d.B; a.n = d.n; a.nm1 = d.nm1;
-  a.yield = d.law ? 1 : 0;
-  a.law = d.law;
+  const BarDesc &d = B.bars[bb.x];
+  const ElemArgs &a = B.bar_ea[bb.x];
+  quad_fill_table(a, Ls);
+  __syncthreads();
   const long long e_end = d.elem0 + d.ne, e_raw = (long long)bb.y + (threadIdx.x >> 2);
-  elem_quad4ax_body<false, false, 64>(a, e_raw, e_end, threadIdx.x, xs);
+  elem_quad4ax_body<false, false, false, 64>(a, e_raw, e_end, threadIdx.x, Ls);
   if (preamble || (st->increment % st->freq) == 0)
   {
     double l = __builtin_inf();

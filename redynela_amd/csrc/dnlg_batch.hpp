@@ -70,11 +70,11 @@ __device__ __forceinline__ void courant_contrib(double l, unsigned long long *cr
 struct ElemPassF64
 {
   static __device__ __forceinline__ void run(const ElemArgs &a, const ElemArgs32 &, long long e0, long long e1,
-                                             double (*xs)[256])
+                                             QuadLds<256> &Ls)
   {
     for (long long base = e0; base < e1; base += 64)
     {
-      elem_quad4ax_body<false, false, 256>(a, base + (threadIdx.x >> 2), e1, threadIdx.x, xs);
+      elem_quad4ax_body<false, false, false, 256>(a, base + (threadIdx.x >> 2), e1, threadIdx.x, Ls);
       __syncthreads();
     }
   }
@@ -83,22 +83,17 @@ struct ElemPassF64
 template <class PASS, bool FE32>
 __global__ __launch_bounds__(256) void k_batch_quad4(BatchArgs B)
 {
-  __shared__ double xs[11][256];
+  __shared__ QuadLds<256> Ls;
   __shared__ double red[256];
   __shared__ int s_go, s_rec;
   const int bar = B.order[blockIdx.x];
   const BarDesc d = B.bars[bar];
   StepState *st = B.states + bar;
-  ElemArgs a = B.ea;
-  a.st = st;
-  a.K = d.K; a.G = d.G; a.A = d.A; a.B = d.B; a.n = d.n; a.nm1 = d.nm1;
-  a.yield = d.law ? 1 : 0;
-  a.law = d.law;
-  ElemArgs32 a32 = B.ea32;
-  a32.st = st;
-  a32.K = (float)d.K; a32.G = (float)d.G; a32.A = (float)d.A; a32.B = (float)d.B; a32.n = (float)d.n;
-  a32.law = d.law;
+  const ElemArgs &a = B.bar_ea[bar];      // in global memory: no per-lane copy
+  const ElemArgs32 &a32 = B.bar_ea32[bar];
   const long long e0 = d.elem0, e1 = d.elem0 + d.ne, n0 = d.node0, n1 = d.node0 + d.nn;
+  quad_fill_table(a, Ls);
+  __syncthreads();
   if (!st->running)
   {
     // betweenTimeBounds false on entry: solve() does nothing (FE/ExplicitSolverCH.C:121-128)
@@ -107,7 +102,7 @@ __global__ __launch_bounds__(256) void k_batch_quad4(BatchArgs B)
   }
   if (threadIdx.x == 0) B.t0[bar] = wall_clock64();
   // solve() preamble: computeInternalForces, computeTimeStep(True)
-  PASS::run(a, a32, e0, e1, xs);
+  PASS::run(a, a32, e0, e1, Ls);
   double crit = bar_courant(a, e0, e1, red);
   if (threadIdx.x == 0) compute_time_step(st, 1, crit);
   __syncthreads();
@@ -137,7 +132,7 @@ __global__ __launch_bounds__(256) void k_batch_quad4(BatchArgs B)
     }
     if (threadIdx.x == 0) init_step(st);
     __syncthreads();
-    PASS::run(a, a32, e0, e1, xs);
+    PASS::run(a, a32, e0, e1, Ls);
     bar_nodal<FE32>(B.na, n0, n1, st);
     __syncthreads();
   }

@@ -39,6 +39,8 @@ struct dnlg_batch
   int *conn = nullptr, *fpos = nullptr, *order_d = nullptr;
   unsigned char *deg = nullptr, *bc = nullptr;
   BarDesc *bars_d = nullptr;
+  ElemArgs *bar_ea = nullptr;
+  ElemArgs32 *bar_ea32 = nullptr;
   StepState *st = nullptr;
   unsigned long long *t0 = nullptr, *t1 = nullptr;
   BatchArgs args{};
@@ -83,7 +85,7 @@ static void bfree(dnlg_batch *b)
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   void *ptrs[] = {b->x, b->v, b->a, b->du, b->d, b->mass, b->me, b->fe, b->S, b->evp, b->p, b->fe32, b->S32,
                   b->evp32, b->p32, b->conn, b->fpos, b->order_d, b->deg, b->bc, b->bars_d, b->st, b->t0, b->t1,
-                  b->blk_e, b->blk_n, b->crit};
+                  b->blk_e, b->blk_n, b->crit, b->bar_ea, b->bar_ea32};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -362,6 +364,38 @@ extern "C" int dnlg_batch_create(int n, const dnlg_mesh *meshes, const dnlg_mate
   na.fe = b->fe;
   na.fe32 = b->fe32;
   na.nn = NN;
+  // per-bar element arguments: the packed arrays with the bar's state and constants
+  {
+    std::vector<ElemArgs> be(n);
+    std::vector<ElemArgs32> be32(n);
+    for (int i = 0; i < n; i++)
+    {
+      const BarDesc &d = b->bars[i];
+      ElemArgs x = ea;
+      x.st = b->st + i;
+      x.K = d.K; x.G = d.G; x.A = d.A; x.B = d.B; x.n = d.n; x.nm1 = d.nm1;
+      x.yield = d.law ? 1 : 0;
+      x.law = d.law;
+      be[i] = x;
+      ElemArgs32 y = e3;
+      y.st = b->st + i;
+      y.K = (float)d.K; y.G = (float)d.G; y.A = (float)d.A; y.B = (float)d.B; y.n = (float)d.n;
+      y.law = d.law;
+      be32[i] = y;
+    }
+    hipError_t e1 = balloc(&b->bar_ea, n), e2 = balloc(&b->bar_ea32, n);
+    if (e1 == hipSuccess) e1 = hipMemcpy(b->bar_ea, be.data(), sizeof(ElemArgs) * n, hipMemcpyHostToDevice);
+    if (e2 == hipSuccess) e2 = hipMemcpy(b->bar_ea32, be32.data(), sizeof(ElemArgs32) * n, hipMemcpyHostToDevice);
+    if (e1 != hipSuccess || e2 != hipSuccess)
+    {
+      g_berr = std::string("HIP error ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2) + " in dnlg_batch_create";
+      bfree(b);
+      delete b;
+      return DNLG_E_HIP;
+    }
+    A.bar_ea = b->bar_ea;
+    A.bar_ea32 = b->bar_ea32;
+  }
   *out = b;
   return DNLG_OK;
 }

@@ -10,7 +10,7 @@ namespace dnl {
 struct ElemPassF32
 {
   static __device__ __forceinline__ void run(const ElemArgs &, const ElemArgs32 &a32, long long e0, long long e1,
-                                             double (*)[256])
+                                             QuadLds<256> &)
   {
     for (long long e = e0 + threadIdx.x; e < e1; e += 256) elem_quad4ax_f32_body(a32, e);
     __syncthreads();
@@ -28,11 +28,8 @@ __global__ __launch_bounds__(64) void k_bstep_elem32(BatchArgs B, const int2 *bl
   const int2 bb = blk[blockIdx.x];
   StepState *st = B.states + bb.x;
   if (!st->running || st->status) return;
-  const BarDesc d = B.bars[bb.x];
-  ElemArgs32 a = B.ea32;
-  a.st = st;
-  a.K = (float)d.K; a.G = (float)d.G; a.A = (float)d.A; a.B = (float)d.B; a.n = (float)d.n;
-  a.law = d.law;
+  const BarDesc &d = B.bars[bb.x];
+  const ElemArgs32 &a = B.bar_ea32[bb.x];
   const long long e_end = d.elem0 + d.ne, e = (long long)bb.y + threadIdx.x;
   if (e < e_end) elem_quad4ax_f32_body(a, e);
   if (preamble || (st->increment % st->freq) == 0)

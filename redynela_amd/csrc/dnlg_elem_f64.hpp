@@ -178,13 +178,58 @@ __device__ __forceinline__ bool stress_update_jc(const ElemArgs &a, const T9 &dE
 }
 
 // ---------------------------------------------------------------------------
+// LDS of the element kernels: the shape-function table of the reference
+// element (filled once per block) and the exchange of the points' data for
+// the nodal force sums: hex8 exchanges J^-1 (9 doubles; the node lane forms
+// dN = dN0 J^-1 itself) so that 3 waves fit per SIMD, quad4 the shape
+// gradients dN themselves.  Rows have a pitch of NT + 1 doubles against bank
+// conflicts.
+// ---------------------------------------------------------------------------
+template <int NT>
+struct HexLds
+{
+  double tab[8 * 8 * 3];  // dN0(ip, k, d)
+  double x[19][NT + 1];   // rows 0..8: J^-1 of the point; 9+c: Sig; 18: w detJ
+};
+template <int NT>
+struct QuadLds
+{
+  double tab[4 * 4 * 3];  // dN0(ip, k, 0..1), N(ip, k)
+  double x[15][NT + 1];   // rows 2k+d: dN(k, d); 8..12: Sig xx, xy, yx, yy, zz; 13: w detJ; 14: r
+};
+
+// fill the tables (all threads of the block; a barrier must follow)
+template <int NT>
+__device__ __forceinline__ void hex_fill_table(const ElemArgs &a, HexLds<NT> &L)
+{
+  for (int i = threadIdx.x; i < 192; i += NT)
+  {
+    double g[3];
+    hex_dn0(a, i / 24, (i / 3) % 8, g);
+    L.tab[i] = g[i % 3];
+  }
+}
+template <int NT>
+__device__ __forceinline__ void quad_fill_table(const ElemArgs &a, QuadLds<NT> &L)
+{
+  for (int i = threadIdx.x; i < 48; i += NT)
+  {
+    const int ip = i / 12, k = (i / 3) % 4, c = i % 3;
+    double g[2];
+    quad_dn0(a, ip, k, g);
+    L.tab[i] = c < 2 ? g[c] : quad_n(a, ip, k);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // hex8 element kernel: lane = 8*element + integration point
 // ---------------------------------------------------------------------------
-// One integration point of one element per lane.  tid indexes the LDS
-// exchange buffer xs[19][W]; lanes 8k..8k+7 of a wave hold one element.
-template <bool SPEED_BC, bool TRACK, int XW>
+// One integration point of one element per lane; lanes 8k..8k+7 of a wave
+// hold one element.  tid indexes the exchange rows of L (the table must be
+// filled).  JC: the Johnson-Cook law (ElemArgs::law == 2) instead of A + B evp^n.
+template <bool SPEED_BC, bool TRACK, bool JC, int NT>
 __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_raw, long long e_end, int tid,
-                                               double (*xs)[XW])
+                                               HexLds<NT> &Ls)
 {
   StepState *st = a.st;
   const int wl = tid & 63;
@@ -204,8 +249,8 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
   {
     const double *xk = a.x + 3 * (long long)nd[k];
     const double x0 = xk[0], x1 = xk[1], x2 = xk[2];
-    double g[3];
-    hex_dn0(a, pt, k, g);
+    const double *tg = Ls.tab + (pt * 8 + k) * 3;
+    const double g[3] = {tg[0], tg[1], tg[2]};
     J.v[0] += g[0] * x0; J.v[1] += g[1] * x0; J.v[2] += g[2] * x0;
     J.v[3] += g[0] * x1; J.v[4] += g[1] * x1; J.v[5] += g[2] * x1;
     J.v[6] += g[0] * x2; J.v[7] += g[1] * x2; J.v[8] += g[2] * x2;
@@ -214,6 +259,8 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
                       (J.v[6] * J.v[4] * J.v[2]) - (J.v[0] * J.v[7] * J.v[5]) - (J.v[3] * J.v[1] * J.v[8]);
   T9 inv;
   if (!inv3(J, inv) && valid) raise_status(st, ST_DETJ, e);
+#pragma unroll
+  for (int c = 0; c < 9; c++) Ls.x[c][tid] = inv.v[c];
 
   // F = I + sum_k dN(k,:) (x) du_k, FE/Element3D.C:182-205
   T9 F = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
@@ -233,8 +280,9 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
         if (m & 4) u2 = sp[2] * st->dt;
       }
     }
-    double g[3], d[3];
-    hex_dn0(a, pt, k, g);
+    const double *tg = Ls.tab + (pt * 8 + k) * 3;
+    const double g[3] = {tg[0], tg[1], tg[2]};
+    double d[3];
     dgemm_row<3>(g, inv.v, d);
     F.v[0] += d[0] * u0; F.v[1] += d[1] * u0; F.v[2] += d[2] * u0;
     F.v[3] += d[0] * u1; F.v[4] += d[1] * u1; F.v[5] += d[2] * u1;
@@ -257,7 +305,7 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
   const double tr = dEps.v[0] + dEps.v[4] + dEps.v[8];
   double Tip = 0.0;
   bool rr_ok;
-  if (a.law == 2)
+  if (JC)  // law 2, a separate instance: its registers do not weigh on law 1
   {
     Tip = a.T[ip];
     rr_ok = stress_update_jc(a, dEps, S, evp, Tip, st->dt, TRACK ? &epl : nullptr);
@@ -286,17 +334,17 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
 #pragma unroll
     for (int c = 0; c < 9; c++) a.S[c * nipt + ip] = S.v[c];
     a.evp[ip] = evp;
-    if (a.law == 2) a.T[ip] = Tip;
+    if (JC) a.T[ip] = Tip;
     if (pt == 0) a.p[e] = p;
     if (TRACK)
 #pragma unroll
       for (int c = 0; c < 9; c++) { a.eps[c * nipt + ip] = eps.v[c]; a.epsplas[c * nipt + ip] = epl.v[c]; }
   }
 
-  // exchange J^-1, Sig and w detJ of the 8 points of the element
+  // exchange Sig and w detJ of the 8 points of the element (dN is in place)
 #pragma unroll
-  for (int c = 0; c < 9; c++) { xs[c][tid] = inv.v[c]; xs[9 + c][tid] = Sig.v[c]; }
-  xs[18][tid] = WxdJ;
+  for (int c = 0; c < 9; c++) Ls.x[9 + c][tid] = Sig.v[c];
+  Ls.x[18][tid] = WxdJ;
   __syncthreads();
 
   // nodal force of local node L = pt, accumulated point by point as
@@ -307,12 +355,16 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
   for (int q = 0; q < 8; q++)
   {
     const int src = lbase + q;
-    double iq[9], sq[9];
+    double sq[9];
 #pragma unroll
-    for (int c = 0; c < 9; c++) { iq[c] = xs[c][src]; sq[c] = xs[9 + c][src]; }
-    const double W = xs[18][src];
-    double g[3], d[3];
-    hex_dn0(a, q, L, g);
+    for (int c = 0; c < 9; c++) sq[c] = Ls.x[9 + c][src];
+    const double W = Ls.x[18][src];
+    double iq[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) iq[c] = Ls.x[c][src];
+    const double *tg = Ls.tab + (q * 8 + L) * 3;
+    const double g[3] = {tg[0], tg[1], tg[2]};
+    double d[3];
     dgemm_row<3>(g, iq, d);
 #pragma unroll
     for (int j = 0; j < 3; j++) f0 -= d[j] * sq[3 * j + 0] * W;
@@ -331,9 +383,9 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
 // ---------------------------------------------------------------------------
 // axisymmetric quad4 element kernel: lane = 4*element + integration point
 // ---------------------------------------------------------------------------
-template <bool SPEED_BC, bool TRACK, int XW>
+template <bool SPEED_BC, bool TRACK, bool JC, int NT>
 __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e_raw, long long e_end, int tid,
-                                                  double (*xs)[XW])
+                                                  QuadLds<NT> &Ls)
 {
   StepState *st = a.st;
   const int wl = tid & 63;
@@ -353,14 +405,14 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
   {
     const double *xk = a.x + 3 * (long long)nd[k];
     const double x0 = xk[0], x1 = xk[1];
-    double g[2];
-    quad_dn0(a, pt, k, g);
+    const double *tg = Ls.tab + (pt * 4 + k) * 3;
+    const double g[2] = {tg[0], tg[1]};
     J0 += g[0] * x0; J1 += g[1] * x0;
     J2 += g[0] * x1; J3 += g[1] * x1;
   }
   // getRadius_atIntPoint, FE/Element2D.C:233-246
 #pragma unroll
-  for (int k = 0; k < 4; k++) Rr += quad_n(a, pt, k) * a.x[3 * (long long)nd[k]];
+  for (int k = 0; k < 4; k++) Rr += Ls.tab[(pt * 4 + k) * 3 + 2] * a.x[3 * (long long)nd[k]];
   const double detJ = (J0 * J3) - (J1 * J2);
   if (detJ < 0. && valid) raise_status(st, ST_DETJ, e);
   const double inv[4] = {J3 / detJ, -J1 / detJ, -J2 / detJ, J0 / detJ};
@@ -383,12 +435,15 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
         if (m & 2) u1 = sp[1] * st->dt;
       }
     }
-    double g[2], d[2];
-    quad_dn0(a, pt, k, g);
+    const double *tg = Ls.tab + (pt * 4 + k) * 3;
+    const double g[2] = {tg[0], tg[1]};
+    double d[2];
     dgemm_row<2>(g, inv, d);
+    Ls.x[2 * k][tid] = d[0];
+    Ls.x[2 * k + 1][tid] = d[1];
     F.v[0] += d[0] * u0; F.v[1] += d[1] * u0;
     F.v[3] += d[0] * u1; F.v[4] += d[1] * u1;
-    vr += quad_n(a, pt, k) * u0;
+    vr += tg[2] * u0;
   }
   F.v[8] += vr / Rr;
 
@@ -407,7 +462,7 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
   const double tr = dEps.v[0] + dEps.v[4] + dEps.v[8];
   double Tip = 0.0;
   bool rr_ok;
-  if (a.law == 2)
+  if (JC)  // law 2, a separate instance: its registers do not weigh on law 1
   {
     Tip = a.T[ip];
     rr_ok = stress_update_jc(a, dEps, S, evp, Tip, st->dt, TRACK ? &epl : nullptr);
@@ -433,17 +488,15 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
 #pragma unroll
     for (int c = 0; c < 9; c++) a.S[c * nipt + ip] = S.v[c];
     a.evp[ip] = evp;
-    if (a.law == 2) a.T[ip] = Tip;
+    if (JC) a.T[ip] = Tip;
     if (pt == 0) a.p[e] = p;
     if (TRACK)
 #pragma unroll
       for (int c = 0; c < 9; c++) { a.eps[c * nipt + ip] = eps.v[c]; a.epsplas[c * nipt + ip] = epl.v[c]; }
   }
 
-#pragma unroll
-  for (int c = 0; c < 4; c++) xs[c][tid] = inv[c];
-  xs[4][tid] = Sig.v[0]; xs[5][tid] = Sig.v[1]; xs[6][tid] = Sig.v[3]; xs[7][tid] = Sig.v[4];
-  xs[8][tid] = Sig.v[8]; xs[9][tid] = WxdJ; xs[10][tid] = Rr;
+  Ls.x[8][tid] = Sig.v[0]; Ls.x[9][tid] = Sig.v[1]; Ls.x[10][tid] = Sig.v[3]; Ls.x[11][tid] = Sig.v[4];
+  Ls.x[12][tid] = Sig.v[8]; Ls.x[13][tid] = WxdJ; Ls.x[14][tid] = Rr;
   __syncthreads();
 
   // FE/Element.C:178-212: in-plane terms, then the hoop term of this point
@@ -453,17 +506,14 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
   for (int q = 0; q < 4; q++)
   {
     const int src = lbase + q;
-    const double iq[4] = {xs[0][src], xs[1][src], xs[2][src], xs[3][src]};
-    const double s00 = xs[4][src], s01 = xs[5][src], s10 = xs[6][src], s11 = xs[7][src], s22 = xs[8][src];
-    const double W = xs[9][src], Rq = xs[10][src];
-    double g[2], d[2];
-    quad_dn0(a, q, L, g);
-    dgemm_row<2>(g, iq, d);
+    const double s00 = Ls.x[8][src], s01 = Ls.x[9][src], s10 = Ls.x[10][src], s11 = Ls.x[11][src];
+    const double s22 = Ls.x[12][src], W = Ls.x[13][src], Rq = Ls.x[14][src];
+    const double d[2] = {Ls.x[2 * L][src], Ls.x[2 * L + 1][src]};
     f0 -= d[0] * s00 * W;
     f0 -= d[1] * s10 * W;
     f1 -= d[0] * s01 * W;
     f1 -= d[1] * s11 * W;
-    f0 -= (quad_n(a, q, L) * s22 / Rq) * W;
+    f0 -= (Ls.tab[(q * 4 + L) * 3 + 2] * s22 / Rq) * W;
   }
   if (valid)
   {

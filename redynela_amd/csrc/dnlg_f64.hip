@@ -21,22 +21,28 @@
 
 namespace dnl {
 
-template <bool SPEED_BC, bool TRACK>
-__global__ __launch_bounds__(64) void k_elem_hex8(ElemArgs a)
+// 3 waves per SIMD for the production instances (168 VGPRs, a few spilled);
+// the strain-tracking instances are not held to it
+template <bool SPEED_BC, bool TRACK, bool JC>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TRACK ? 1 : 3))) void k_elem_hex8(ElemArgs a)
 {
   if (a.check_running && !a.st->running) return;
-  __shared__ double xs[19][64];
+  __shared__ HexLds<64> Ls;
+  hex_fill_table(a, Ls);
+  __syncthreads();
   const long long e_raw = ((long long)blockIdx.x * 64 + threadIdx.x) >> 3;
-  elem_hex8_body<SPEED_BC, TRACK, 64>(a, e_raw, a.ne, threadIdx.x, xs);
+  elem_hex8_body<SPEED_BC, TRACK, JC, 64>(a, e_raw, a.ne, threadIdx.x, Ls);
 }
 
-template <bool SPEED_BC, bool TRACK>
+template <bool SPEED_BC, bool TRACK, bool JC>
 __global__ __launch_bounds__(64) void k_elem_quad4ax(ElemArgs a)
 {
   if (a.check_running && !a.st->running) return;
-  __shared__ double xs[11][64];
+  __shared__ QuadLds<64> Ls;
+  quad_fill_table(a, Ls);
+  __syncthreads();
   const long long e_raw = ((long long)blockIdx.x * 64 + threadIdx.x) >> 2;
-  elem_quad4ax_body<SPEED_BC, TRACK, 64>(a, e_raw, a.ne, threadIdx.x, xs);
+  elem_quad4ax_body<SPEED_BC, TRACK, JC, 64>(a, e_raw, a.ne, threadIdx.x, Ls);
 }
 
 // ---------------------------------------------------------------------------
@@ -323,7 +329,12 @@ void launch_elem_f64(int dim, const ElemArgs &a, bool speed_bc, hipStream_t s)
   const bool tr = a.eps != nullptr;
   const long long lanes = a.ne * (dim == 3 ? 8 : 4);
   const unsigned grid = (unsigned)((lanes + 63) / 64);
-#define DNL_ELEM(K, B, T) hipLaunchKernelGGL((K<B, T>), dim3(grid), dim3(64), 0, s, a)
+#define DNL_ELEM(K, B, T)                                                                      \
+  do                                                                                           \
+  {                                                                                            \
+    if (a.law == 2) hipLaunchKernelGGL((K<B, T, true>), dim3(grid), dim3(64), 0, s, a);       \
+    else hipLaunchKernelGGL((K<B, T, false>), dim3(grid), dim3(64), 0, s, a);                 \
+  } while (0)
   if (dim == 3)
   {
     if (speed_bc) { if (tr) DNL_ELEM(k_elem_hex8, true, true); else DNL_ELEM(k_elem_hex8, true, false); }

@@ -114,6 +114,8 @@ struct BatchArgs
   unsigned long long *t0, *t1;  // wall clock at the start / end of each bar
   long long max_steps;
   unsigned long long *crit;  // stepped engine: per-bar Courant minimum (bits of a positive double)
+  const ElemArgs *bar_ea;    // per bar: ea with the bar's state pointer and material constants
+  const ElemArgs32 *bar_ea32;
 };
 
 // adjoint of one increment (dnlg_adjoint.hip): scalar adjoints and kernel arguments

@@ -328,8 +328,12 @@ DNL_HD bool inv3(const T9 &J, T9 &out)
   return true;
 }
 
-// one row of dN = dN0 * invJ (dgemm NN order, LP/Blas/dgemm.c: for each
-// column j, C(i,j) = 0 then += (1*B(l,j)) * A(i,l) for l with B(l,j) != 0)
+// one row of dN = dN0 * invJ in the dgemm NN order (LP/Blas/dgemm.c: for
+// each column j, C(i,j) = 0 then += (1*B(l,j)) * A(i,l) for l with
+// B(l,j) != 0).  The zero test is value-neutral and left out: the dN0 entries
+// are finite and non-zero, so a skipped term would add +-0 to a partial sum
+// that is never -0 (it starts at +0 and sums of non-zero terms that cancel
+// give +0), and 1*b is b.
 template <int D>
 DNL_HD void dgemm_row(const double *g, const double *inv, double *out)
 {
@@ -338,11 +342,7 @@ DNL_HD void dgemm_row(const double *g, const double *inv, double *out)
   {
     double c = 0.;
 #pragma unroll
-    for (int l = 0; l < D; l++)
-    {
-      const double b = inv[l * D + j];
-      c = (b != 0.) ? c + (1.0 * b) * g[l] : c;
-    }
+    for (int l = 0; l < D; l++) c = c + inv[l * D + j] * g[l];
     out[j] = c;
   }
 }
