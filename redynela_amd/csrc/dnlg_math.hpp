@@ -156,6 +156,9 @@ DNL_HD bool polar_lnU(const T9 &F, T9 &U, T9 &R)
 {
   T9 FF = mul_at(F, F);
   T9 Rot = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+  // the sweep; a lane that has converged leaves the loop and waits, so the
+  // closing step below runs once per wave instead of once per exit iteration
+  bool conv = false;
   for (int it = 1; it < 30; it++)
   {
     int q = (dabs(FF.v[1]) > dabs(FF.v[2]) ? 1 : 2);
@@ -165,25 +168,8 @@ DNL_HD bool polar_lnU(const T9 &F, T9 &U, T9 &R)
     const double fpq = (p == 0 ? (q == 1 ? FF.v[1] : FF.v[2]) : FF.v[5]);
     if (dabs(fpq) < kPolarPrec)
     {
-      T9 u, Ru;
-      const double lam[3] = {FF.v[0], FF.v[4], FF.v[8]};
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-      {
-        const double v0 = Rot.v[c], v1 = Rot.v[3 + c], v2 = Rot.v[6 + c];
-        const double ext[9] = {v0 * v0, v0 * v1, v0 * v2, v1 * v0, v1 * v1, v1 * v2, v2 * v0, v2 * v1, v2 * v2};
-        const double a = det_log(lam[c]) / 2.;
-        const double b = 1. / sqrt(lam[c]);
-#pragma unroll
-        for (int k = 0; k < 9; k++)
-        {
-          if (c == 0) { u.v[k] = a * ext[k]; Ru.v[k] = b * ext[k]; }
-          else { u.v[k] += a * ext[k]; Ru.v[k] += b * ext[k]; }
-        }
-      }
-      U = u;
-      R = mul(F, Ru);
-      return true;
+      conv = true;
+      break;
     }
     const double fqq = (q == 1 ? FF.v[4] : FF.v[8]);
     const double fpp = (p == 0 ? FF.v[0] : FF.v[4]);
@@ -216,7 +202,28 @@ DNL_HD bool polar_lnU(const T9 &F, T9 &U, T9 &R)
     FF = {{u0, u1, u2, u1, u4, u5, u2, u5, u8}};
     Rot = mul(Rot, r);
   }
-  return false;
+  if (!conv) return false;
+  // U = sum_c ln(lam_c)/2 v_c v_c^T, R = F sum_c lam_c^-1/2 v_c v_c^T with
+  // v_c the columns of the accumulated rotation (LA/Tensor2.C:1180-1197)
+  T9 u, Ru;
+  const double lam[3] = {FF.v[0], FF.v[4], FF.v[8]};
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+  {
+    const double v0 = Rot.v[c], v1 = Rot.v[3 + c], v2 = Rot.v[6 + c];
+    const double ext[9] = {v0 * v0, v0 * v1, v0 * v2, v1 * v0, v1 * v1, v1 * v2, v2 * v0, v2 * v1, v2 * v2};
+    const double a = det_log(lam[c]) / 2.;
+    const double b = 1. / sqrt(lam[c]);
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+    {
+      if (c == 0) { u.v[k] = a * ext[k]; Ru.v[k] = b * ext[k]; }
+      else { u.v[k] += a * ext[k]; Ru.v[k] += b * ext[k]; }
+    }
+  }
+  U = u;
+  R = mul(F, Ru);
+  return true;
 }
 
 // Matrix::inverse of a 3x3 (LA/Matrix.C:1653-1723): LAPACK dgetf2 (partial
