@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 HIST_B, CONN_B, NODE_B, ELEM_NODE_READ_B = 1024.0, 32.0, 276.0, 48.0
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4  # wave64 VALU instructions per ns: 256 CUs x 4 SIMDs, 4 cycles each at 2.4 GHz
 
 
 def main():
@@ -44,7 +45,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-bars leg (FP64 and mixed)")
-    ap.add_argument("--batch-engine", default="auto", choices=["auto", "persistent", "stepped"])
+    ap.add_argument("--batch-engine", default="auto", choices=["auto", "persistent", "stepped", "stepped-plain"])
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--precision-report", action="store_true",
                     help="only the O-grid Johnson-Cook bar (4,000 elements per section) in both forms")
@@ -115,7 +116,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args)
-    traffic = profiled_traffic()
+    traffic = profiled("traffic_bytes_per_launch")
+    valu_insts = profiled("valu_insts_per_launch")
 
     out = {
         "metric": "element-steps/s, 1M hex8 FP64 vs FP32 (err vs FP64 oracle); batched bars/s",
@@ -138,6 +140,11 @@ def main():
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic, "avg_ms": avg_el * 1e3, "launches": n_el,
                      "algorithmic_bytes_per_launch": elem_bytes},
+        # the bound that applies to the bit-exact FP64 kernel: VALU issue, one
+        # wave64 instruction per 4 cycles on each of 1,024 SIMDs at 2.4 GHz
+        "valu_roofline": None if valu_insts is None else {
+            "insts_per_launch": valu_insts, "achieved_Ginst_s": valu_insts / avg_el / 1e9,
+            "peak_Ginst_s": VALU_PEAK_GINST, "frac": valu_insts / avg_el / 1e9 / VALU_PEAK_GINST},
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
                           "nodal_kernel_avg_ms": avg_nd * 1e3},
         "fp32_form": fp32,
@@ -270,13 +277,14 @@ def precision_report(layers=1000, t_end=60e-6):
     return out
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the element kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel)."""
+def profiled(key):
+    """A per-launch figure of the element kernel from the committed rocprofv3
+    summary: traffic_bytes_per_launch (FETCH_SIZE / WRITE_SIZE, calibrated
+    against a copy kernel) or valu_insts_per_launch (SQ_INSTS_VALU)."""
     p = os.path.join(ROOT, "profiles", "k_elem_hex8_fp64_1M.md")
     try:
         for line in open(p):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":")[1])
     except OSError:
         pass

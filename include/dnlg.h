@@ -205,6 +205,7 @@ int dnlg_batch_destroy(dnlg_batch *b);
 #define DNLG_BATCH_AUTO 0
 #define DNLG_BATCH_PERSISTENT 1
 #define DNLG_BATCH_STEPPED 2
+#define DNLG_BATCH_STEPPED_PLAIN 3 /* STEPPED launched increment by increment, without graphs (for tracing tools) */
 int dnlg_batch_set_engine(dnlg_batch *b, int engine);
 /* runs every domain to up_time (or its end_time); at most max_steps increments
  * per domain (DNLG_E_STATE if a domain needs more).  wall_us[i] (optional) gets

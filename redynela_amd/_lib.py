@@ -28,7 +28,7 @@ EXPORTS = [
     "dnlg_batch_last_error", "dnlg_batch_taylor_params", "dnlg_batch_set_engine",
     "dnlg_adjoint_forward", "dnlg_adjoint_backward",
 ]
-BATCH_AUTO, BATCH_PERSISTENT, BATCH_STEPPED = 0, 1, 2
+BATCH_AUTO, BATCH_PERSISTENT, BATCH_STEPPED, BATCH_STEPPED_PLAIN = 0, 1, 2, 3
 
 
 class Mesh(C.Structure):

@@ -81,9 +81,11 @@ class Batch:
             raise DnlgError("%s: %s" % (_lib.ERRORS.get(rc, rc), _lib.lib().dnlg_batch_last_error(self.h).decode()))
 
     def set_engine(self, engine):
-        """'auto', 'persistent' (one workgroup per domain, one launch) or
-        'stepped' (one increment of every domain per launch group)."""
-        e = {"auto": _lib.BATCH_AUTO, "persistent": _lib.BATCH_PERSISTENT, "stepped": _lib.BATCH_STEPPED}[engine]
+        """'auto', 'persistent' (one workgroup per domain, one launch),
+        'stepped' (one increment of every domain per launch group, replayed
+        from a captured graph) or 'stepped-plain' (the same without graphs)."""
+        e = {"auto": _lib.BATCH_AUTO, "persistent": _lib.BATCH_PERSISTENT, "stepped": _lib.BATCH_STEPPED,
+             "stepped-plain": _lib.BATCH_STEPPED_PLAIN}[engine]
         self._check(_lib.lib().dnlg_batch_set_engine(self.h, e))
 
     def solve(self, upTime, max_steps=200000):

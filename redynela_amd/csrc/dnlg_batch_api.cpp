@@ -430,7 +430,7 @@ static int compact_tables(dnlg_batch *b, const std::vector<StepState> &st, bool 
     be.insert(be.end(), b->bar_be[i].begin(), b->bar_be[i].end());
     bn.insert(bn.end(), b->bar_bn[i].begin(), b->bar_bn[i].end());
   }
-  if (!force && b->graph && 4 * (long long)be.size() > 3 * (long long)b->graph_nblk_e) return DNLG_OK;
+  if (!force && 4 * (long long)be.size() > 3 * (long long)b->graph_nblk_e) return DNLG_OK;
   b->nblk_e = (int)be.size();
   b->nblk_n = (int)bn.size();
   if (be.empty()) return DNLG_OK;
@@ -442,6 +442,9 @@ static int compact_tables(dnlg_batch *b, const std::vector<StepState> &st, bool 
     (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
   }
+  b->graph_nblk_e = b->nblk_e;
+  b->graph_nblk_n = b->nblk_n;
+  if (b->engine == DNLG_BATCH_STEPPED_PLAIN) return DNLG_OK;
   hipGraph_t g;
   BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   for (int k = 0; k < kGraphSteps; k++) issue_stepped(b, 0);
@@ -449,8 +452,6 @@ static int compact_tables(dnlg_batch *b, const std::vector<StepState> &st, bool 
   const hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   BHIP(e);
-  b->graph_nblk_e = b->nblk_e;
-  b->graph_nblk_n = b->nblk_n;
   return DNLG_OK;
 }
 
@@ -469,7 +470,8 @@ static int solve_stepped(dnlg_batch *b, long long max_steps, std::vector<StepSta
   {
     const long long n = std::min(chunk, max_steps - done);
     long long k = 0;
-    for (; k + kGraphSteps <= n; k += kGraphSteps) BHIP(hipGraphLaunch(b->graph, b->stream));
+    if (b->graph)
+      for (; k + kGraphSteps <= n; k += kGraphSteps) BHIP(hipGraphLaunch(b->graph, b->stream));
     for (; k < n; k++) issue_stepped(b, 0);
     BHIP(hipGetLastError());
     done += n;
@@ -488,7 +490,12 @@ static int solve_stepped(dnlg_batch *b, long long max_steps, std::vector<StepSta
 
 extern "C" int dnlg_batch_set_engine(dnlg_batch *b, int engine)
 {
-  if (!b || engine < DNLG_BATCH_AUTO || engine > DNLG_BATCH_STEPPED) return DNLG_E_ARG;
+  if (!b || engine < DNLG_BATCH_AUTO || engine > DNLG_BATCH_STEPPED_PLAIN) return DNLG_E_ARG;
+  if (b->graph)
+  {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
   b->engine = engine;
   return DNLG_OK;
 }

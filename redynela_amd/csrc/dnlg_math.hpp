@@ -154,63 +154,72 @@ DNL_HD double sumsq(const T9 &a)
 // Jacobi sweep does not converge in 29 rotations.
 DNL_HD bool polar_lnU(const T9 &F, T9 &U, T9 &R)
 {
-  T9 FF = mul_at(F, F);
-  T9 Rot = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+  const T9 FF0 = mul_at(F, F);
+  // the six distinct entries of the symmetric F^T F (the dense formula below
+  // keeps it symmetric bit for bit) and the accumulated rotation
+  double f0 = FF0.v[0], f1 = FF0.v[1], f2 = FF0.v[2], f4 = FF0.v[4], f5 = FF0.v[5], f8 = FF0.v[8];
+  double r0[3] = {1, 0, 0}, r1[3] = {0, 1, 0}, r2[3] = {0, 0, 1};  // columns of Rot
   // the sweep; a lane that has converged leaves the loop and waits, so the
   // closing step below runs once per wave instead of once per exit iteration
   bool conv = false;
   for (int it = 1; it < 30; it++)
   {
-    int q = (dabs(FF.v[1]) > dabs(FF.v[2]) ? 1 : 2);
-    const double fq = (q == 1 ? FF.v[1] : FF.v[2]);
-    int p = (dabs(fq) > dabs(FF.v[5]) ? 0 : 1);
+    int q = (dabs(f1) > dabs(f2) ? 1 : 2);
+    const double fq = (q == 1 ? f1 : f2);
+    const int p = (dabs(fq) > dabs(f5) ? 0 : 1);
     if (p == 1) q = 2;
-    const double fpq = (p == 0 ? (q == 1 ? FF.v[1] : FF.v[2]) : FF.v[5]);
-    if (dabs(fpq) < kPolarPrec)
+    const bool c01 = p == 0 && q == 1, c02 = p == 0 && q == 2, c12 = p == 1;
+    // entries relabelled by (p, q, m), m the third index
+    const double Apq = c01 ? f1 : (c02 ? f2 : f5);
+    if (dabs(Apq) < kPolarPrec)
     {
       conv = true;
       break;
     }
-    const double fqq = (q == 1 ? FF.v[4] : FF.v[8]);
-    const double fpp = (p == 0 ? FF.v[0] : FF.v[4]);
-    const double alpha = (fqq - fpp) / (2. * fpq);
+    const double App = c12 ? f4 : f0, Aqq = c01 ? f4 : f8;
+    const double Apm = c01 ? f2 : f1, Aqm = c12 ? f2 : f5;
+    const double alpha = (Aqq - App) / (2. * Apq);
     const double t = (alpha != 0. ? 1. / (alpha + sqrt(alpha * alpha + 1.) * (alpha > 0. ? 1. : -1.)) : 1.);
     const double Ct = 1. / (sqrt(1. + t * t));
-    const double St = t * Ct;
-    // rotation with R(p,q) = S, R(q,p) = -S, R(p,p) = R(q,q) = C
-    T9 r = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
-    if (p == 0 && q == 1) { r.v[1] = St; r.v[3] = -St; r.v[0] = Ct; r.v[4] = Ct; }
-    else if (p == 0) { r.v[2] = St; r.v[6] = -St; r.v[0] = Ct; r.v[8] = Ct; }
-    else { r.v[5] = St; r.v[7] = -St; r.v[4] = Ct; r.v[8] = Ct; }
-    const double *R_ = r.v, *F_ = FF.v;
-    // R^T FF R, dense formula of LA/Tensor2.C:1216-1239
-    const double c0 = F_[0] * R_[0] + F_[3] * R_[3] + F_[6] * R_[6];
-    const double c1 = F_[1] * R_[0] + F_[4] * R_[3] + F_[7] * R_[6];
-    const double c2 = F_[2] * R_[0] + F_[5] * R_[3] + F_[8] * R_[6];
-    const double d0 = F_[0] * R_[1] + F_[3] * R_[4] + F_[6] * R_[7];
-    const double d1 = F_[1] * R_[1] + F_[4] * R_[4] + F_[7] * R_[7];
-    const double d2 = F_[2] * R_[1] + F_[5] * R_[4] + F_[8] * R_[7];
-    const double e0 = F_[0] * R_[2] + F_[3] * R_[5] + F_[6] * R_[8];
-    const double e1 = F_[1] * R_[2] + F_[4] * R_[5] + F_[7] * R_[8];
-    const double e2 = F_[2] * R_[2] + F_[5] * R_[5] + F_[8] * R_[8];
-    const double u0 = R_[0] * c0 + R_[3] * c1 + R_[6] * c2;
-    const double u1 = R_[1] * c0 + R_[4] * c1 + c2 * R_[7];
-    const double u2 = R_[2] * c0 + R_[5] * c1 + c2 * R_[8];
-    const double u4 = R_[1] * d0 + R_[4] * d1 + R_[7] * d2;
-    const double u5 = R_[2] * d0 + R_[5] * d1 + d2 * R_[8];
-    const double u8 = R_[2] * e0 + R_[5] * e1 + R_[8] * e2;
-    FF = {{u0, u1, u2, u1, u4, u5, u2, u5, u8}};
-    Rot = mul(Rot, r);
+    const double St = t * Ct, mSt = -St;
+    // R^T FF R of LA/Tensor2.C:1216-1239 with the rotation's zero and unit
+    // entries left out: each kept product and the order of each sum are the
+    // dense formula's (a dropped term adds +-0, a unit factor is exact)
+    const double wpp = App * Ct + Apq * mSt, wpq = Apq * Ct + Aqq * mSt;
+    const double wqp = App * St + Apq * Ct, wqq = Apq * St + Aqq * Ct;
+    const double nApp = Ct * wpp + mSt * wpq;
+    const double nAqq = St * wqp + Ct * wqq;
+    const double nApq = St * wpp + Ct * wpq;
+    const double nApm = Apm * Ct + Aqm * mSt;
+    const double nAqm = Apm * St + Aqm * Ct;
+    const double Amm = c01 ? f8 : (c02 ? f4 : f0);
+    f0 = c12 ? f0 : nApp;
+    f4 = c01 ? nAqq : (c02 ? Amm : nApp);
+    f8 = c01 ? Amm : nAqq;
+    f1 = c01 ? nApq : nApm;
+    f2 = c01 ? nApm : (c02 ? nApq : nAqm);
+    f5 = c12 ? nApq : nAqm;
+    // Rot = Rot r: columns p and q mix, column m is unchanged
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+    {
+      const double cp = p == 0 ? r0[i] : r1[i], cq = q == 1 ? r1[i] : r2[i];
+      const double np = cp * Ct + cq * mSt, nq = cp * St + cq * Ct;
+      r0[i] = p == 0 ? np : r0[i];
+      r1[i] = c01 ? nq : (c02 ? r1[i] : np);
+      r2[i] = q == 2 ? nq : r2[i];
+    }
   }
   if (!conv) return false;
   // U = sum_c ln(lam_c)/2 v_c v_c^T, R = F sum_c lam_c^-1/2 v_c v_c^T with
   // v_c the columns of the accumulated rotation (LA/Tensor2.C:1180-1197)
   T9 u, Ru;
-  const double lam[3] = {FF.v[0], FF.v[4], FF.v[8]};
+  const double lam[3] = {f0, f4, f8};
 #pragma unroll
   for (int c = 0; c < 3; c++)
   {
-    const double v0 = Rot.v[c], v1 = Rot.v[3 + c], v2 = Rot.v[6 + c];
+    const double *col = c == 0 ? r0 : (c == 1 ? r1 : r2);
+    const double v0 = col[0], v1 = col[1], v2 = col[2];
     const double ext[9] = {v0 * v0, v0 * v1, v0 * v2, v1 * v0, v1 * v1, v1 * v2, v2 * v0, v2 * v1, v2 * v2};
     const double a = det_log(lam[c]) / 2.;
     const double b = 1. / sqrt(lam[c]);

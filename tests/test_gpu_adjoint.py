@@ -69,10 +69,13 @@ def test_adjoint_forward_is_the_production_stepping():
     d.begin(1e300)
     d.step(150)
     assert np.array_equal(x, d.coords)
-    # the backward leaves the post-forward state in place
-    p.backward(np.ones((x.shape[0], 2)))
+    # the backward leaves the post-forward state in place, and a second
+    # backward over the same checkpoints gives the same gradient
+    g1 = p.backward(np.ones((x.shape[0], 2)))
     assert np.array_equal(p.dom.coords, d.coords)
     assert p.dom.time()["increment"] == 150
+    g2 = p.backward(np.ones((x.shape[0], 2)))
+    assert np.array_equal(g1, g2)
     p.close()
     d.close()
 

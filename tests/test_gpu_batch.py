@@ -111,19 +111,20 @@ def test_persistent_engine_equals_stepped(workload, form):
     from redynela_amd import _lib as L
     meshes, mats = workload
     idx = list(range(0, 512, 21))
-    say("case: %d bars, %s, persistent vs stepped engine, to 40 us" % (len(idx), form))
+    say("case: %d bars, %s, persistent vs stepped engine (graph and plain), to 40 us" % (len(idx), form))
     out = []
-    for eng in ("persistent", "stepped"):
+    for eng in ("persistent", "stepped", "stepped-plain"):
         b = R.Batch([meshes[i] for i in idx], [mats[i] for i in idx], solver(form))
         b.set_engine(eng)
         w, ms = b.solve(T_END)
         say("  %s: %.2f ms" % (eng, ms))
         out.append([(b.time(k)["increment"], b.coords(k), b.evp(k), b.field(k, L.F_S)) for k in range(len(idx))])
         b.close()
-    for p, q in zip(*out):
-        assert p[0] == q[0]
-        for u, v in zip(p[1:], q[1:]):
-            assert np.array_equal(u, v)
+    for other in out[1:]:
+        for p, q in zip(out[0], other):
+            assert p[0] == q[0]
+            for u, v in zip(p[1:], q[1:]):
+                assert np.array_equal(u, v)
 
 
 def test_batch_mixed_tracks_fp64(workload, fp64_batch):
