@@ -47,6 +47,7 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="only the batched-bars leg (FP64 and mixed)")
     ap.add_argument("--batch-engine", default="auto", choices=["auto", "persistent", "stepped", "stepped-plain"])
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-shard", action="store_true")
     ap.add_argument("--precision-report", action="store_true",
                     help="only the O-grid Johnson-Cook bar (4,000 elements per section) in both forms")
     ap.add_argument("--precision-layers", type=int, default=1000)
@@ -110,6 +111,9 @@ def main():
     batched = None
     if not args.no_batch:
         batched = {form: batched_bars(form, engine=args.batch_engine) for form in ("fp64", "mixed")}
+    shard = None
+    if not args.no_shard:
+        shard = shard_leg(args)
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg()
@@ -150,6 +154,7 @@ def main():
         "fp32_form": fp32,
         "batched": batched,
         "adjoint": adjoint,
+        "shard": shard,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -213,6 +218,35 @@ def batched_bars(form, n=512, t_end=40e-6, engine="auto"):
             "increments_min_max": [int(steps.min()), int(steps.max())],
             "increments_p50_p99": [float(np.percentile(steps, 50)), float(np.percentile(steps, 99))],
             "elements_total": int(ne.sum()), "most_increments": slow}
+
+
+def shard_leg(args, layers=64):
+    """One GPU's share of the 1M bar split in axial slabs over 8 GPUs: 63
+    layers plus a one-layer halo (128,000 elements), same material and speed,
+    FP64; the element and nodal kernels at that shape."""
+    import redynela_amd as R
+    from redynela_amd import _lib as L
+    mesh = R.mesh.ogrid_c1(layers)
+    ne = mesh["conn"].shape[0]
+    mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
+    sol = R.ExplicitSolverCH()
+    sol.setDissipation(1.0)
+    d = R.Domain(mesh, mat, sol)
+    d.set_option(L.OPT_STORE_FINT, 0)
+    d.begin(1.0)
+    d.step_async(args.warmup)
+    d.synchronize()
+    d.enable_kernel_timing(True)
+    t0 = time.perf_counter()
+    d.step_async(args.steps)
+    d.synchronize()
+    el = time.perf_counter() - t0
+    n_el, ms_el = d.kernel_stats(0)
+    n_nd, ms_nd = d.kernel_stats(1)
+    d.close()
+    return {"workload": "hex8 O-grid 2,000 el/section x %d layers (1/8 of the 1M bar + 1 halo layer), FP64" % layers,
+            "elements": ne, "element_steps_per_s": ne * args.steps / el, "ms_per_step": el / args.steps * 1e3,
+            "element_kernel_ms": ms_el / max(n_el, 1), "nodal_kernel_ms": ms_nd / max(n_nd, 1)}
 
 
 def adjoint_leg():
