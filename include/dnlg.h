@@ -112,6 +112,7 @@ typedef struct dnlg_handle dnlg_handle;
 #define DNLG_F_EPSPLAS 11 /* [9][n_elems*nip] IntegrationPoint::EpsPlas (if tracked)*/
 #define DNLG_F_FE 12      /* [n_elems*nen][dim] element nodal forces of the last pass */
 #define DNLG_F_TEMP 13    /* [n_elems*nip] integration-point temperature (hardening = 2)  */
+#define DNLG_F_DEPS 14    /* [9][n_elems*nip] IntegrationPoint::dEps of the last pass (if tracked) */
 /* In DNLG_FORM_MIXED, DNLG_F_S is [6][nip][n_elems] (xx,yy,zz,xy,yz,xz) and
  * DNLG_F_EVP / DNLG_F_TEMP are [nip][n_elems]; values are widened to double. */
 

@@ -1181,7 +1181,7 @@ long dnlo_solve(dnlo *m, double upTime)
 }
 
 /* ---- state access ---- */
-enum { F_X = 0, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_SIG, F_X0, F_FE, F_TEMP };
+enum { F_X = 0, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_SIG, F_X0, F_FE, F_TEMP, F_DEPS };
 long dnlo_get(const dnlo *m, int field, double *out)
 {
   long n = 0;
@@ -1204,6 +1204,7 @@ long dnlo_get(const dnlo *m, int field, double *out)
   case F_SIG: src = m->Sig; n = m->ne * m->nip * 9; break;
   case F_FE: src = m->Fe; n = m->ne * m->nen * m->dim; break;
   case F_TEMP: src = m->T; n = m->T ? m->ne * m->nip : 0; break;
+  case F_DEPS: src = m->dEps; n = m->ne * m->nip * 9; break;
   default: return -1;
   }
   if (out && n) memcpy(out, src, sizeof(double) * n);

@@ -15,7 +15,7 @@ DNLG_OK = 0
 ERRORS = {1: "DNLG_E_ARG", 2: "DNLG_E_POLAR_NOCONV", 3: "DNLG_E_RR_NOCONV", 4: "DNLG_E_NEG_DETJ",
           5: "DNLG_E_HIP", 6: "DNLG_E_STATE"}
 
-F_COORDS, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_FE, F_TEMP = range(14)
+F_COORDS, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_FE, F_TEMP, F_DEPS = range(15)
 OPT_KINETIC_ENERGY, OPT_STORE_FINT, OPT_TRACK_DISP, OPT_TRACK_STRAIN = 1, 2, 3, 4
 
 # every symbol include/dnlg.h declares

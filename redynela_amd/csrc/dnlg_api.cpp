@@ -57,6 +57,7 @@ static void fill_elem_args(dnlg_handle *h)
   a.p = h->p;
   a.eps = h->track_strain ? h->eps : nullptr;
   a.epsplas = h->track_strain ? h->epsplas : nullptr;
+  a.deps = h->track_strain ? h->deps : nullptr;
   a.fe = h->fe;
   a.fpos = h->fpos;
   a.bcs = h->bcs;
@@ -155,7 +156,7 @@ static void free_all(dnlg_handle *h)
 {
   adj_free(h->adj);
   h->adj = nullptr;
-  void *ptrs[] = {h->x, h->v, h->a, h->du, h->d, h->mass, h->fint, h->fe, h->S, h->evp, h->p, h->eps, h->epsplas,
+  void *ptrs[] = {h->x, h->v, h->a, h->du, h->d, h->mass, h->fint, h->fe, h->S, h->evp, h->p, h->eps, h->epsplas, h->deps,
                   h->me, h->part, h->ke_part, h->bcs_v, h->conn, h->fpos, h->deg, h->bc, h->bcs, h->st, h->T,
                   h->S32, h->evp32, h->T32, h->p32, h->fe32};
   for (void *q : ptrs)
@@ -387,8 +388,10 @@ extern "C" int dnlg_set_option(dnlg_handle *h, int option, int value)
       const long long nipt = h->ne * h->nip;
       DNLG_HIP(h, dalloc(&h->eps, nipt * 9));
       DNLG_HIP(h, dalloc(&h->epsplas, nipt * 9));
+      DNLG_HIP(h, dalloc(&h->deps, nipt * 9));
       DNLG_HIP(h, hipMemset(h->eps, 0, sizeof(double) * nipt * 9));
       DNLG_HIP(h, hipMemset(h->epsplas, 0, sizeof(double) * nipt * 9));
+      DNLG_HIP(h, hipMemset(h->deps, 0, sizeof(double) * nipt * 9));
     }
     h->track_strain = value != 0;
     fill_elem_args(h);
@@ -634,7 +637,7 @@ extern "C" long long dnlg_field_size(dnlg_handle *h, int field)
   case DNLG_F_S: return nipt * (mx ? 6 : 9);
   case DNLG_F_EVP: return nipt;
   case DNLG_F_P: return h->ne;
-  case DNLG_F_EPS: case DNLG_F_EPSPLAS: return h->eps ? nipt * 9 : 0;
+  case DNLG_F_EPS: case DNLG_F_EPSPLAS: case DNLG_F_DEPS: return h->eps ? nipt * 9 : 0;
   case DNLG_F_FE: return h->ne * h->nen * h->dim;
   case DNLG_F_TEMP: return (h->T || h->T32) ? nipt : 0;
   default: return -1;
@@ -657,6 +660,7 @@ static double *field_ptr(dnlg_handle *h, int field)
   case DNLG_F_P: return h->p;
   case DNLG_F_EPS: return h->eps;
   case DNLG_F_EPSPLAS: return h->epsplas;
+  case DNLG_F_DEPS: return h->deps;
   case DNLG_F_FE: return h->fe;
   case DNLG_F_TEMP: return h->T;
   default: return nullptr;

@@ -338,7 +338,12 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
     if (pt == 0) a.p[e] = p;
     if (TRACK)
 #pragma unroll
-      for (int c = 0; c < 9; c++) { a.eps[c * nipt + ip] = eps.v[c]; a.epsplas[c * nipt + ip] = epl.v[c]; }
+      for (int c = 0; c < 9; c++)
+      {
+        a.eps[c * nipt + ip] = eps.v[c];
+        a.epsplas[c * nipt + ip] = epl.v[c];
+        a.deps[c * nipt + ip] = dEps.v[c];
+      }
   }
 
   // exchange Sig and w detJ of the 8 points of the element (dN is in place)
@@ -492,7 +497,12 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
     if (pt == 0) a.p[e] = p;
     if (TRACK)
 #pragma unroll
-      for (int c = 0; c < 9; c++) { a.eps[c * nipt + ip] = eps.v[c]; a.epsplas[c * nipt + ip] = epl.v[c]; }
+      for (int c = 0; c < 9; c++)
+      {
+        a.eps[c * nipt + ip] = eps.v[c];
+        a.epsplas[c * nipt + ip] = epl.v[c];
+        a.deps[c * nipt + ip] = dEps.v[c];
+      }
   }
 
   Ls.x[8][tid] = Sig.v[0]; Ls.x[9][tid] = Sig.v[1]; Ls.x[10][tid] = Sig.v[3]; Ls.x[11][tid] = Sig.v[4];

@@ -20,7 +20,7 @@ struct dnlg_handle
   // device buffers
   double *x = nullptr, *v = nullptr, *a = nullptr, *du = nullptr, *d = nullptr, *mass = nullptr;
   double *fint = nullptr, *fe = nullptr, *S = nullptr, *evp = nullptr, *p = nullptr;
-  double *eps = nullptr, *epsplas = nullptr, *me = nullptr, *part = nullptr, *ke_part = nullptr;
+  double *eps = nullptr, *epsplas = nullptr, *deps = nullptr, *me = nullptr, *part = nullptr, *ke_part = nullptr;
   double *bcs_v = nullptr;
   int *conn = nullptr, *fpos = nullptr;
   unsigned char *deg = nullptr;

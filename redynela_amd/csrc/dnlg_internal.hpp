@@ -41,6 +41,7 @@ struct ElemArgs
   double *p;             // [ne] pressure (equal at every IP of an element)
   double *eps;           // [9][ne*nip] or null
   double *epsplas;       // [9][ne*nip] or null
+  double *deps;          // [9][ne*nip] or null: IntegrationPoint::dEps of the last pass
   double *fe;            // [maxdeg][nn][dim] element nodal forces, slot-major per node (ELL)
   const int *fpos;       // [ne*nen] slot*nn + node of each (element, local node)
   const unsigned char *bcs;  // speed-BC component mask per node or null

@@ -11,7 +11,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
-F_X, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_SIG, F_X0, F_FE, F_TEMP = range(16)
+F_X, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_SIG, F_X0, F_FE, F_TEMP, F_DEPS = range(17)
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -113,6 +113,11 @@ class Model:
             j = mat["jc"]
             L.dnlo_set_johnson_cook(self.h, j["C"], j["epsd0"], j["m"], j["T0"], j["Tm"], j["cp"], j["tq"])
         L.dnlo_set_ke(self.h, 1 if ke else 0)
+        if mesh.get("speed_mask") is not None:  # constant BoundarySpeed nodes
+            sm = np.asarray(mesh["speed_mask"])
+            sp = np.ascontiguousarray(mesh["speed"], dtype=np.float64)
+            for i in np.nonzero(sm)[0]:
+                L.dnlo_set_speed_bc(self.h, int(i), int(sm[i]), sp[i].copy())
         st = L.dnlo_init(self.h)
         if st:
             raise RuntimeError("oracle init status %d" % st)
@@ -150,3 +155,26 @@ class Model:
         if getattr(self, "h", None):
             lib().dnlo_free(self.h)
             self.h = None
+
+
+def from_deck(d, mathmode=1):
+    """An oracle model of a redynela_amd.deck.Deck (one material, Courant step)."""
+    mesh, mat, sol = d.model()
+    h = mat.hardening
+    m = dict(E=mat.young, nu=mat.poisson, rho=mat.density, A=h.A if h else 0.0, B=h.B if h else 0.0,
+             n=h.n if h else 1.0, **({} if h else {"yield": 0}))
+    s = dict(rho_b=sol.rho_b, safety=sol.safety, freq=sol.freq, end_time=sol.stop)
+    return Model(mesh, m, s, mathmode)
+
+
+def result_blocks(o, density, T0=0.0):
+    """node and point records of an oracle model in the result-file layout
+    (redynela_amd.results), from the oracle's own fields."""
+    from redynela_amd import results as RF
+    nn = o.nn
+    vec = lambda f: o.get(f).reshape(nn, 3)
+    tm = o.time()
+    nodes = RF.node_block(vec(F_X), vec(F_V), vec(F_A), vec(F_DU), vec(F_D), tm["increment"], density, T0)
+    ten = lambda f: o.get(f).reshape(-1, 9)
+    ips = RF.ip_block(o.get(F_P), o.get(F_EVP), ten(F_S), ten(F_EPS), ten(F_DEPS), ten(F_EPSPLAS))
+    return tm["t"], nodes, ips

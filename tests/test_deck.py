@@ -1,0 +1,170 @@
+"""Deck reader (`.dyn` interpreter subset) and the Structure::solve save loop,
+on the CPU.
+
+The reader is pinned three ways: the reference's own Taylor sample deck
+(Samples/Taylor/Taylor_Axi_Parametric.dyn, read in place when the reference
+tree is present) must build exactly the mesh of the axisymmetric bar mesher
+(which itself restates that deck's loops and is checked against the oracle
+runs); a deck written for these tests builds the small bar; and the
+expression rules of the reference's grammar (Sources/Libraries/interpretor/
+InputDyn.y:104-110, 1560-1590) are checked on expressions whose value
+differs under C's rules.
+"""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SMALL = os.path.join(HERE, "decks", "taylor_axi_small.dyn")
+HEX = os.path.join(HERE, "decks", "hex_block.dyn")
+SAMPLE = "/root/reference/Samples/Taylor/Taylor_Axi_Parametric.dyn"
+
+
+def _same_mesh(m, r):
+    for k in ("x", "conn", "bc", "v0"):
+        assert m[k].shape == r[k].shape, k
+        assert np.array_equal(m[k], r[k]), k
+    assert m["dim"] == r["dim"]
+
+
+def test_small_deck_builds_the_axi_bar():
+    from redynela_amd import deck, mesh
+    d = deck.load(SMALL)
+    m, mat, sol = d.model()
+    _same_mesh(m, mesh.taylor_axi(4, 10, 3.2e-3, 6.48e-3, 227.0))
+    assert (mat.young, mat.poisson, mat.density) == (115e9, 0.3, 8930.0)
+    h = mat.hardening
+    assert (h.kind, h.A, h.B, h.n) == ("IsoHardElastoplastic", 400e6, 100e6, 1.0)
+    assert (sol.start, sol.stop, sol.rho_b, sol.safety, sol.freq) == (0.0, 6e-6, 1.0, 0.9, 10)
+    assert d.save == (0.0, 6e-6, 2e-6)
+    assert d.result_base() == SMALL[:-4]
+
+
+@pytest.mark.skipif(not os.path.exists(SAMPLE), reason="reference tree not present")
+def test_reference_sample_deck_builds_the_axi_bar():
+    from redynela_amd import deck, mesh
+    d = deck.load(SAMPLE)
+    m, mat, sol = d.model()
+    _same_mesh(m, mesh.taylor_axi(25, 250, 3.2e-3, 32.4e-3, 227.0))
+    assert (mat.young, mat.poisson, mat.density) == (117e9, 0.35, 8930.0)
+    assert sol.rho_b == 0.8182 and sol.stop == 80e-6
+    assert d.save[2] == 80e-6 / 20
+
+
+def test_hex_deck():
+    from redynela_amd import deck
+    d = deck.load(HEX)
+    m, mat, sol = d.model()
+    assert m["dim"] == 3 and m["conn"].shape == (2 * 2 * 3, 8) and m["x"].shape == (3 * 3 * 4, 3)
+    # the first element's nodes in ElHex8n3D order: the base face counter-clockwise, then the top face
+    x = m["x"][m["conn"][0]]
+    np.testing.assert_array_equal(x[:, 2], [0, 0, 0, 0, 1e-3, 1e-3, 1e-3, 1e-3])
+    np.testing.assert_array_equal(x[:4, :2], [[0, 0], [1e-3, 0], [1e-3, 1e-3], [0, 1e-3]])
+    # restrained base (z), initial speed everywhere, constant speed on the top face
+    assert np.all(m["bc"][m["x"][:, 2] == 0] == 4)
+    assert np.all(m["v0"][:, 2] == -150.0)
+    top = m["x"][:, 2] == 3e-3
+    assert np.all(m["speed_mask"][top] == 7) and np.all(m["speed_mask"][~top] == 0)
+    np.testing.assert_array_equal(m["speed"][top], np.tile([0, 0, -150.0], (top.sum(), 1)))
+
+
+def test_node_and_element_numbering_order():
+    """a Domain keeps nodes and elements sorted by number (FE/Domain.C:1548-1592):
+    a deck creating them in reverse order builds the same mesh"""
+    from redynela_amd import deck
+    fwd = """Structure s; ElQua4nAx el; s.setDefaultElement(el);
+    s.createNode(1,0,0,0); s.createNode(2,1,0,0); s.createNode(3,1,1,0); s.createNode(4,0,1,0);
+    s.createNode(5,2,0,0); s.createNode(6,2,1,0);
+    s.createElement(1,1,2,3,4); s.createElement(2,2,5,6,3);"""
+    rev = """Structure s; ElQua4nAx el; s.setDefaultElement(el);
+    s.createNode(6,2,1,0); s.createNode(5,2,0,0); s.createNode(4,0,1,0); s.createNode(3,1,1,0);
+    s.createNode(2,1,0,0); s.createNode(1,0,0,0);
+    s.createElement(2,2,5,6,3); s.createElement(1,1,2,3,4);"""
+    tail = """Material m; m.setYoung(1e9); m.setPoisson(.3); m.setDensity(1000);
+    ElementSet all; all.add(1,2); s.attachMaterialToElements(m, all);
+    NodeSet ns; ns.add(5); BoundaryRestrain b; b.set(1,0,0); s.attachConstantBC(b, ns);
+    ExplicitSolverCH so; so.setTimes(0,1e-6); s.addSolver(so);"""
+    a, _, _ = deck.load(fwd + tail, text=True).model()
+    b, _, _ = deck.load(rev + tail, text=True).model()
+    for k in ("x", "conn", "bc", "v0"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["bc"][4] == 1 and a["bc"].sum() == 1
+    with pytest.raises(deck.DeckError, match="Node 2 already exists"):
+        deck.load("Structure s; s.createNode(2,0,0,0); s.createNode(2,1,0,0);", text=True)
+
+
+def test_expression_rules():
+    """values that differ between the reference grammar and C precedence"""
+    from redynela_amd import deck
+    d = deck.load("""
+    Real a = !0 + 1;       // NOT takes the whole sum: !(0 + 1)
+    Real b = !0 < 1;       // ... but not a comparison: (!0) < 1
+    Real c = 0 && 0 < 1;   // && and < share one left-associative level: (0 && 0) < 1
+    Real d = 2^3^2;        // ^ is left associative
+    Real e = -2^2;         // unary minus binds tightest
+    Real f = 1 + !1 + 1;   // 1 + !(1 + 1)
+    Real g = int(-2.7);    // truncation
+    Real h = round(2.5);   // int(x + 0.5)
+    Real i = log(1000);    // base 10
+    Real j = ln(1);
+    Real k = sin(30);      // degrees
+    Real l = 7 > 3 == 1;
+    Real m = 10; m += 5; m *= 2; m -= 1; m /= 29;
+    Real n = 0; Real q; for (q = 0; q < 5; q++) n += q;
+    Real r = 0; if (n == 10) r = 1;
+    """, text=True)
+    v = d.vars
+    assert (v["a"], v["b"], v["c"], v["d"], v["e"], v["f"]) == (0.0, 0.0, 1.0, 64.0, 4.0, 1.0)
+    assert (v["g"], v["h"], v["i"], v["j"], v["l"]) == (-2.0, 3.0, 3.0, 0.0, 1.0)
+    assert abs(v["k"] - 0.5) < 1e-15
+    assert (v["m"], v["n"], v["r"]) == (1.0, 10.0, 1.0)
+    r = deck.load("#pragma radians\nReal s = sin(PI/2);", text=True)
+    assert r.vars["s"] == 1.0
+
+
+def test_sets_inclusive_ranges():
+    """NodeSet::add(first, last, increment) includes last (FE/Select.C:151-212)"""
+    from redynela_amd import deck
+    d = deck.load("Structure s; Real i; for (i = 1; i <= 10; i++) s.createNode(i, i, 0, 0);"
+                  "NodeSet a; a.add(1, 10, 3); NodeSet b; b.add(4, 6); NodeSet c; c.add(9);", text=True)
+    assert d.vars["a"].items == [1, 4, 7, 10]
+    assert d.vars["b"].items == [4, 5, 6]
+    assert d.vars["c"].items == [9]
+    with pytest.raises(deck.DeckError, match="Node 11 not exist"):
+        deck.load("Structure s; s.createNode(1, 0, 0, 0); NodeSet a; a.add(1, 11, 10);", text=True)
+
+
+def test_unsupported_constructs_raise():
+    from redynela_amd import deck
+    with pytest.raises(deck.DeckError, match="DiscreteFunction is not supported"):
+        deck.load("DiscreteFunction f;", text=True)
+    with pytest.raises(deck.DeckError, match="unknown name zz"):
+        deck.load("Real a = zz + 1;", text=True)
+    with pytest.raises(deck.DeckError, match="defines no elements"):
+        deck.load("Structure s;", text=True).model()
+
+
+def test_structure_solve_save_loop():
+    """Structure::solve with saveResults (FE/Structure.C:786-799, 833-883):
+    saves at start (or every) and each `every` after, nothing at time 0,
+    and the last save only if the accumulated save time is reached"""
+    from redynela_amd import deck
+    calls = []
+
+    def solve(t, stop=6e-6):
+        calls.append(t)
+        return min(t, stop)
+
+    saved = deck.structure_solve(6e-6, (0.0, 6e-6, 2e-6), solve, lambda t: None)
+    assert saved == [2e-6, 4e-6, 6e-6] and calls == [2e-6, 4e-6, 6e-6]
+    # a start time other than 0 is the first save
+    saved = deck.structure_solve(6e-6, (1e-6, 6e-6, 2e-6), solve, lambda t: None)
+    assert saved == [1e-6, 1e-6 + 2e-6, 1e-6 + 2e-6 + 2e-6]  # nextSaveTime += saveTime, accumulated
+    # 0.1 + 0.1 + 0.1 > 0.3: the reference's accumulated save time overshoots
+    # the end, the domain stops at its end time and no third file is written
+    saved = deck.structure_solve(0.3, (0.0, 0.3, 0.1), lambda t: min(t, 0.3), lambda t: None)
+    assert saved == [0.1, 0.2]
+    # a save period of 0 asks Domain::solve for a time already reached
+    with pytest.raises(deck.DeckError, match="Unable to solve problem upto time =  0.000E\\+00"):
+        deck.structure_solve(1.0, (0.0, 1.0, 0.0), lambda t: t, lambda t: None)

@@ -155,6 +155,7 @@ def test_compute_internal_forces_random_increment():
         assert same(tens(g, L.F_S), o.get(O.F_S))
         assert same(tens(g, L.F_EPS), o.get(O.F_EPS))
         assert same(tens(g, L.F_EPSPLAS), o.get(O.F_EPSPLAS))
+        assert same(tens(g, L.F_DEPS), o.get(O.F_DEPS))
         assert same(g.field(L.F_P), o.get(O.F_P)[::g.nip])
 
 

@@ -1,0 +1,105 @@
+"""Result files (`<base>.NNN.bin`) in the reference's byte layout, on the CPU.
+
+The layout is restated from the reference's writers (FE/Structure.C:936-1011,
+FE/Domain.C:1407-1431, FE/Node.C:156-162, FE/NodalField.C:102-117,
+FE/IntegrationPoint.C:155-168); a hand-built one-node, one-point file pins
+the order of every double.  A run of the oracle over the small test deck
+then writes its saves, which must read back as the oracle's own state, with
+Sig equal to the oracle's Sig and ro/T following the swap parity.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SMALL = os.path.join(HERE, "decks", "taylor_axi_small.dyn")
+
+
+def test_layout_one_node_one_point(tmp_path):
+    from redynela_amd import results as RF
+    nodes = RF.node_block(np.array([[1.0, 2.0, 3.0]]), np.array([[4.0, 5.0, 6.0]]), np.array([[7.0, 8.0, 9.0]]),
+                          np.array([[10.0, 11.0, 12.0]]), np.array([[13.0, 14.0, 15.0]]), 0, 8930.0, 293.0)
+    S = np.arange(101.0, 110.0)[None, :]
+    ips = RF.ip_block(np.array([-2.0]), np.array([0.5]), S, S + 100, S + 200, S + 300)
+    p = str(tmp_path / "one.001.bin")
+    RF.write(p, 1.5e-6, nodes, ips)
+    raw = np.fromfile(p, dtype="<f8")
+    sig = S[0] + np.array([-2.0, 0, 0, 0, -2.0, 0, 0, 0, -2.0])
+    want = np.concatenate([
+        [3, 1, 0, 1.02, 0, 0, 1.5e-6],            # Structure: version, check, time
+        [3, 1, 0, 1.02, 0, 0, 1.5e-6],            # Domain: version, check, time
+        [1, 2, 3, 8930.0, 0, 0, 0, 293.0],        # coords, ro, dro, e, de, T
+        [4, 5, 6, 7, 8, 9, 0, 0, 0, 10, 11, 12, 13, 14, 15],  # v, a, fe, du, d
+        [69, 0, 0],                               # separator
+        [-2.0, 0.5], S[0], sig, S[0] + 100, S[0] + 200, S[0] + 300,  # p evp S Sig Eps dEps EpsPlas
+        [69, 0, 0], [69, 0, 0]])
+    assert raw.size == 93 and np.array_equal(raw, want)
+    # after an odd number of increments the Current nodal field is the cleared one
+    odd = RF.node_block(np.zeros((1, 3)), *(np.zeros((1, 3)),) * 4, 7, 8930.0, 293.0)
+    assert odd[0, 3] == 0.0 and odd[0, 7] == 0.0
+    r = RF.read(p, 1, 1)
+    assert r["time"] == 1.5e-6 and r["domain_time"] == 1.5e-6
+    assert np.array_equal(r["nodes"], nodes) and np.array_equal(r["ips"], ips)
+
+
+def test_read_checks_marks(tmp_path):
+    from redynela_amd import results as RF
+    from redynela_amd._lib import DnlgError
+    nodes = np.zeros((2, RF.NODE_DOUBLES))
+    ips = np.zeros((3, RF.IP_DOUBLES))
+    p = str(tmp_path / "x.001.bin")
+    RF.write(p, 0.0, nodes, ips)
+    with pytest.raises(DnlgError, match="doubles"):
+        RF.read(p, 2, 4)
+    raw = np.fromfile(p, dtype="<f8")
+    bad = raw.copy()
+    bad[3] = 1.01
+    bad.tofile(p)
+    with pytest.raises(DnlgError, match="not compatible"):
+        RF.read(p, 2, 3)
+    bad = raw.copy()
+    bad[14 + 2 * RF.NODE_DOUBLES] = 70.0
+    bad.tofile(p)
+    with pytest.raises(DnlgError, match="truncated"):
+        RF.read(p, 2, 3)
+
+
+def test_oracle_deck_run_reads_back(tmp_path):
+    from redynela_amd import deck
+    from redynela_amd import results as RF
+    d = deck.load(SMALL)
+    mesh, mat, _ = d.model()
+    o = O.from_deck(d)
+    base = str(tmp_path / "taylor_axi_small")
+    writer_state = []
+
+    def save(t):
+        tt, nodes, ips = O.result_blocks(o, mat.density, mat.init_temp)
+        assert tt == t
+        path = "%s.%03d.bin" % (base, len(writer_state) + 1)
+        RF.write(path, t, nodes, ips)
+        writer_state.append((path, t, o.time()["increment"], o.get(O.F_X).copy(), o.get(O.F_SIG).copy(),
+                             o.get(O.F_EVP).copy()))
+
+    def solve(t):
+        assert o.solve(t) >= 0
+        return o.time()["t"]
+
+    times = deck.structure_solve(d.solver.stop, d.save, solve, save)
+    assert len(times) == 3 and [os.path.basename(w[0]) for w in writer_state] == [
+        "taylor_axi_small.001.bin", "taylor_axi_small.002.bin", "taylor_axi_small.003.bin"]
+    nn, nipt = mesh["x"].shape[0], mesh["conn"].shape[0] * 4
+    for path, t, inc, x, sig, evp in writer_state:
+        r = RF.read(path, nn, nipt)
+        assert r["time"] == t
+        assert np.array_equal(r["nodes"][:, 0:3].reshape(-1), x)
+        # Sig = S + p I as the oracle forms it, to the sign of every zero
+        assert np.array_equal(np.ascontiguousarray(r["ips"][:, 11:20]).reshape(-1).view(np.int64), sig.view(np.int64))
+        assert np.array_equal(r["ips"][:, 1], evp)
+        ro = r["nodes"][:, 3]
+        assert np.all(ro == (mat.density if inc % 2 == 0 else 0.0))
+        assert np.all(r["nodes"][:, [4, 5, 6, 7, 14, 15, 16]] == 0.0)
+    assert np.max(writer_state[-1][5]) > 0.0
