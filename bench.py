@@ -73,6 +73,7 @@ def main():
     ne, nn = mesh["conn"].shape[0], mesh["x"].shape[0]
     mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
     sol = R.ExplicitSolverCH()
+    sol.setTimeStepMethod("Courant")
     sol.setDissipation(1.0)
     dom = R.Domain(mesh, mat, sol)
     dom.set_option(L.OPT_STORE_FINT, 0)
@@ -165,6 +166,7 @@ def mixed_form(mesh, args, fp64_value):
     import redynela_amd as R
     mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
     sol = R.ExplicitSolverCH()
+    sol.setTimeStepMethod("Courant")
     sol.setDissipation(1.0)
     sol.setPrecision("mixed")
     dom = R.Domain(mesh, mat, sol)
@@ -194,6 +196,7 @@ def batched_bars(form, n=512, t_end=40e-6, engine="auto"):
     import redynela_amd as R
     meshes, mats = R.batch.taylor_batch(n)
     sol = R.ExplicitSolverCH()
+    sol.setTimeStepMethod("Courant")
     sol.setDissipation(1.0)
     sol.setPrecision(form)
     warm = R.Batch(meshes[:8], mats[:8], sol)  # loads the code object
@@ -230,6 +233,7 @@ def shard_leg(args, layers=64):
     ne = mesh["conn"].shape[0]
     mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
     sol = R.ExplicitSolverCH()
+    sol.setTimeStepMethod("Courant")
     sol.setDissipation(1.0)
     d = R.Domain(mesh, mat, sol)
     d.set_option(L.OPT_STORE_FINT, 0)
@@ -287,6 +291,7 @@ def precision_report(layers=1000, t_end=60e-6):
         h = R.IsoHardJohnsonCook(90e6, 292e6, 0.31, C=0.025, m=1.09, depsp0=1.0, Tm=1356.0, T0=293.0)
         h.enableRateAndThermal(383.0, 0.9)
         sol = R.ExplicitSolverCH()
+        sol.setTimeStepMethod("Courant")
         sol.setDissipation(1.0)
         sol.setPrecision(form)
         d = R.Domain(mesh, R.Material(115e9, 0.3, 8930.0, h), sol)

@@ -32,6 +32,7 @@ extern "C" {
 #define DNLG_E_NEG_DETJ 4       /* negative (2D) or singular (3D) Jacobian, FE/Element2D.C:101-109 */
 #define DNLG_E_HIP 5            /* HIP runtime error                                               */
 #define DNLG_E_STATE 6          /* call out of sequence                                            */
+#define DNLG_E_PULSE_NOCONV 7   /* power iteration > 1000 iterations, FE/Domain.C:936-939          */
 
 /* element families: the reference's ElQua4nAx (FE/ElQua4nAx.C) and ElHex8n3D (FE/ElHex8n3D.C) */
 #define DNLG_DIM_AXI 2
@@ -121,6 +122,11 @@ typedef struct dnlg_handle dnlg_handle;
 #define DNLG_OPT_STORE_FINT 2     /* keep Domain::Fint in device memory every step (default 1)    */
 #define DNLG_OPT_TRACK_DISP 3     /* keep NodalField::disp (default 1)                            */
 #define DNLG_OPT_TRACK_STRAIN 4   /* keep IntegrationPoint Eps, EpsPlas; set before the first step (default 0) */
+#define DNLG_OPT_TIME_STEP_METHOD 5 /* Solver::setTimeStepMethod (FE/Solver.C:312-327): DNLG_TS_COURANT
+                                       (the default of this ABI) or DNLG_TS_PULSE (the default of the
+                                       reference's ExplicitSolverCH, FE/ExplicitSolverCH.C:83)       */
+#define DNLG_TS_COURANT 0
+#define DNLG_TS_PULSE 1
 
 int dnlg_abi_version(void);
 
@@ -157,6 +163,13 @@ long long dnlg_field_size(dnlg_handle *h, int field);
 /* time scalars: out[0] t, [1] dt, [2] increment, [3] computed time step,
  * [4] kinetic energy of the last step, [5] alphaM, [6] beta, [7] gamma, [8] OmegaS */
 int dnlg_get_time(dnlg_handle *h, double *out);
+
+/* Pulse time step (Domain::getMaximumFrequency, FE/Domain.C:816-946; no
+ * accessor in the reference): out[0] calls so far, [1] iterations of the
+ * last call, [2] fmax0 (the last fmax, Domain::fmax0), [3] its convergence
+ * measure.  A step that recomputes the Pulse time step waits on the host
+ * for the power iteration, also under dnlg_step_async. */
+int dnlg_pulse_info(dnlg_handle *h, double *out4);
 
 /* HIP stream of the handle (hipStream_t), for callers that order work on it */
 void *dnlg_get_stream(dnlg_handle *h);

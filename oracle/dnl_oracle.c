@@ -71,6 +71,7 @@
 #define O_E_RR 3     /* FE/Element.C:327-330 */
 #define O_E_DETJ 4   /* FE/Element2D.C:101-109 (2D aborts; 3D only warns) */
 #define O_E_ARG 5
+#define O_E_PULSE 7  /* FE/Domain.C:936-939: power iteration not converged in 1000 iterations */
 
 /* ========================================================================
  * Deterministic math ("det" mode).  Build-defined, no reference counterpart:
@@ -487,6 +488,14 @@ typedef struct
   double rr_prec; /* radial-return stop, 1e-8 as the reference (FE/DefaultValues.h:29) */
   double ke;
   int status;
+  /* time-step method: 0 Courant, 1 Pulse (Solver::setTimeStepMethod, FE/Solver.C:312-327) */
+  int ts_method;
+  /* Domain::getMaximumFrequency state (FE/Domain.h:65-66): the last iterate and
+   * fmax0, kept from one call to the next; Ke per element [ne][nd*nd] */
+  double fmax0;
+  double *ev, *Ke;
+  long pulse_iters; /* iterations of the last call */
+  double pulse_conv; /* its last convergence measure */
 } dnlo;
 
 static double *o_alloc(size_t n)
@@ -1004,12 +1013,215 @@ double dnlo_courant_crit(dnlo *m)
   return crit;
 }
 
-/* Solver::computeTimeStep, FE/Solver.C:220-267 (Courant method) */
-static void o_compute_time_step(dnlo *m, int force)
+/* Material::getHookeMatrix (FE/Material.C:270-331): the threedimensional
+ * 6x6 for hex8, the axisymetric 4x4 for the axisymmetric quad; row-major.
+ * Returns the order. */
+int dnlo_hooke(const dnlo *m, double *C)
+{
+  const double Nu = m->nu, E = m->E;
+  const int nc = m->dim == 3 ? 6 : 4;
+  for (int k = 0; k < nc * nc; k++) C[k] = 0.0;
+  if (nc == 6)
+  {
+    C[0] = 1; C[7] = 1; C[14] = 1;
+    C[21] = (1.0 - 2.0 * Nu) / (2. * (1.0 - Nu));
+    C[28] = (1.0 - 2.0 * Nu) / (2. * (1.0 - Nu));
+    C[35] = (1.0 - 2.0 * Nu) / (2. * (1.0 - Nu));
+    C[1] = C[6] = Nu / (1. - Nu);
+    C[2] = C[12] = Nu / (1. - Nu);
+    C[8] = C[13] = Nu / (1. - Nu);
+  }
+  else
+  {
+    C[0] = 1; C[5] = 1; C[15] = 1;
+    C[4] = Nu / (1. - Nu); C[1] = Nu / (1. - Nu);
+    C[12] = Nu / (1. - Nu); C[13] = Nu / (1. - Nu);
+    C[3] = Nu / (1. - Nu); C[7] = Nu / (1. - Nu);
+    C[10] = (1.0 - 2.0 * Nu) / (2. * (1.0 - Nu));
+  }
+  const double f = (E * (1.0 - Nu) / ((1.0 + Nu) * (1.0 - 2.0 * Nu)));
+  for (int k = 0; k < nc * nc; k++) C[k] *= f; /* Matrix::operator*=, LA/Matrix.C:547-554 */
+  return nc;
+}
+
+/* Element3D::computeElasticStiffnessMatrix (FE/Element3D.C:242-333) and
+ * Element2D::computeElasticStiffnessMatrix for the axisymmetric family
+ * (FE/Element2D.C:129-222), full integration, from the point dN and detJ of
+ * the last computeInternalMatrices.  K is [nd][nd] row-major (the reference
+ * stores column-major; the entries and their sums are the same). */
+void dnlo_element_stiffness(const dnlo *m, long e, double *K)
+{
+  double C[36];
+  const int nc = dnlo_hooke(m, C);
+  const int nen = m->nen, dim = m->dim, nd = nen * dim;
+#define Cm(r, c) C[(r) * nc + (c)]
+#define Km(r, c) K[(r) * nd + (c)]
+  for (int k = 0; k < nd * nd; k++) K[k] = 0.0;
+  double CB[6][24];
+  for (int pt = 0; pt < m->nip; pt++)
+  {
+    const long ip = e * m->nip + pt;
+    const double *dN = m->dN + ip * nd;
+    double W = 1.0 * m->detJ[ip];
+    if (dim == 3)
+    {
+      for (int i = 0; i < 8; i++)
+      {
+        const int I = 3 * i;
+        const double d0 = dN[I], d1 = dN[I + 1], d2 = dN[I + 2];
+        for (int r = 0; r < 6; r++)
+        {
+          CB[r][I] = (Cm(r, 0) * d0 + Cm(r, 3) * d1 + Cm(r, 5) * d2);
+          CB[r][I + 1] = (Cm(r, 1) * d1 + Cm(r, 3) * d0 + Cm(r, 4) * d2);
+          CB[r][I + 2] = (Cm(r, 2) * d2 + Cm(r, 4) * d1 + Cm(r, 5) * d0);
+        }
+      }
+      for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++)
+        {
+          const int I = 3 * i, J = 3 * j;
+          const double a0 = dN[I], a1 = dN[I + 1], a2 = dN[I + 2];
+          for (int c = 0; c < 3; c++)
+          {
+            Km(I, J + c) += (a0 * CB[0][J + c] + a1 * CB[3][J + c] + a2 * CB[5][J + c]) * W;
+          }
+          for (int c = 0; c < 3; c++)
+          {
+            Km(I + 1, J + c) += (a1 * CB[1][J + c] + a0 * CB[3][J + c] + a2 * CB[4][J + c]) * W;
+          }
+          for (int c = 0; c < 3; c++)
+          {
+            Km(I + 2, J + c) += (a2 * CB[2][J + c] + a1 * CB[4][J + c] + a0 * CB[5][J + c]) * W;
+          }
+        }
+    }
+    else
+    {
+      const double R = o_radius(m, e, pt);
+      W *= 2 * O_PI * R;
+      for (int i = 0; i < 4; i++)
+      {
+        const int I = 2 * i;
+        const double d0 = dN[I], d1 = dN[I + 1], Ni = Q4_N[pt][i];
+        for (int r = 0; r < 3; r++)
+        {
+          CB[r][I] = (Cm(r, 0) * d0 + Cm(r, 2) * d1);
+          CB[r][I + 1] = (Cm(r, 1) * d1 + Cm(r, 2) * d0);
+        }
+        CB[3][I] = (Cm(3, 0) * d0 + Cm(3, 2) * d1);
+        CB[3][I + 1] = (Cm(3, 1) * d1 + Cm(3, 2) * d0);
+        for (int r = 0; r < 4; r++) CB[r][I] += Cm(r, 3) * Ni / R;
+      }
+      for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+        {
+          const int I = 2 * i, J = 2 * j;
+          const double a0 = dN[I], a1 = dN[I + 1], Ni = Q4_N[pt][i];
+          Km(I, J) += (a0 * CB[0][J] + a1 * CB[2][J]) * W;
+          Km(I, J + 1) += (a0 * CB[0][J + 1] + a1 * CB[2][J + 1]) * W;
+          Km(I + 1, J) += (a1 * CB[1][J] + a0 * CB[2][J]) * W;
+          Km(I + 1, J + 1) += (a1 * CB[1][J + 1] + a0 * CB[2][J + 1]) * W;
+          Km(I, J) += CB[3][J] * Ni / R * W;
+          Km(I, J + 1) += CB[3][J + 1] * Ni / R * W;
+        }
+    }
+  }
+#undef Cm
+#undef Km
+}
+
+/* Domain::getMaximumFrequency (FE/Domain.C:816-946), the power iteration on
+ * M^-1 K with the assembly order of one thread: element by element in index
+ * order into a zeroed vector (the per-thread vectors of 843-893 summed for
+ * threads = 1).  The start vector, ev(0) = 1, ev(i) = ev(i-1) - 2/(noddl-1),
+ * is built when fmax0 is still 0 (FE/Domain.C:849-856); later calls start
+ * from the last iterate and compare the first fmax with the last one.  Ke x
+ * is LAPACK's reference dgemv ('N', beta 0: y = 0, then y += x_j K(:,j) for
+ * j in order; LA/Matrix.C:875-900), ev /= fmax multiplies by 1/fmax
+ * (LA/Vector.C:652-659).  Returns O_OK and sqrt(fmax), or O_E_PULSE when
+ * iteration > 1000 (FE/Domain.C:936-939, limits FE/DefaultValues.h:35-36). */
+int dnlo_max_frequency(dnlo *m, double *out)
+{
+  const int nen = m->nen, dim = m->dim, nd = nen * dim;
+  const long noddl = (long)dim * m->nn;
+  if (!m->Ke) m->Ke = o_alloc((size_t)m->ne * nd * nd);
+#pragma omp parallel for schedule(static)
+  for (long e = 0; e < m->ne; e++) dnlo_element_stiffness(m, e, m->Ke + e * nd * nd);
+  if (m->fmax0 == 0 || !m->ev)
+  {
+    if (!m->ev) m->ev = o_alloc(noddl);
+    m->ev[0] = 1.0;
+    for (long i = 1; i < noddl; i++) m->ev[i] = m->ev[i - 1] - 2. / (noddl - 1);
+  }
+  double *ev0 = o_alloc(noddl), *acc = o_alloc(noddl);
+  double fmax = 0.0;
+  long iteration = 0;
+  int ok = 0, st = O_OK;
+  while (!ok)
+  {
+    iteration++;
+    memcpy(ev0, m->ev, sizeof(double) * noddl);
+    for (long k = 0; k < noddl; k++) acc[k] = 0.;
+    for (long e = 0; e < m->ne; e++)
+    {
+      const int *cn = m->conn + e * nen;
+      const double *K = m->Ke + e * nd * nd;
+      double loc[24], y[24];
+      for (int J = 0; J < nd; J++) loc[J] = ev0[(long)cn[J / dim] * dim + J % dim];
+      for (int I = 0; I < nd; I++) y[I] = 0.0;
+      for (int J = 0; J < nd; J++)
+      {
+        const double t = 1.0 * loc[J];
+        for (int I = 0; I < nd; I++) y[I] = y[I] + t * K[I * nd + J];
+      }
+      for (int I = 0; I < nd; I++) acc[(long)cn[I / dim] * dim + I % dim] += y[I];
+    }
+    fmax = 0.0;
+    for (long i = 0; i < noddl; i++)
+    {
+      double val = 0.0;
+      val += acc[i];
+      val /= m->mass[i / dim];
+      if (fabs(val) > fmax) fmax = fabs(val);
+      m->ev[i] = val;
+    }
+    const double inv = 1.0 / fmax;
+    for (long i = 0; i < noddl; i++) m->ev[i] *= inv;
+    const double convergence = fabs(fmax - m->fmax0) / fmax;
+    m->pulse_conv = convergence;
+    if (convergence < 1e-4) ok = 1;
+    m->fmax0 = fmax;
+    if (iteration > 1000)
+    {
+      st = O_E_PULSE;
+      break;
+    }
+  }
+  free(ev0);
+  free(acc);
+  m->pulse_iters = iteration;
+  if (out) *out = sqrt(fmax);
+  return st;
+}
+
+void dnlo_set_ts_method(dnlo *m, int method) { m->ts_method = method; }
+void dnlo_compute_mass(dnlo *m) { o_mass(m); }
+long dnlo_pulse_iterations(const dnlo *m) { return m->pulse_iters; }
+double dnlo_pulse_fmax0(const dnlo *m) { return m->fmax0; }
+
+/* Solver::computeTimeStep, FE/Solver.C:220-267 (Courant or Pulse) */
+static int o_compute_time_step(dnlo *m, int force)
 {
   if (force || (m->increment % m->freq == 0))
   {
-    double fmax = 2.0 / dnlo_courant_crit(m);
+    double fmax;
+    if (m->ts_method == 1)
+    {
+      int st = dnlo_max_frequency(m, &fmax);
+      if (st) return st;
+    }
+    else
+      fmax = 2.0 / dnlo_courant_crit(m);
     m->dt = m->safety * m->omegaS / fmax;
     m->lastTimeStepComputationDate = m->timeStepComputationDate;
     m->lastTimeStep = m->computedTimeStep;
@@ -1022,6 +1234,7 @@ static void o_compute_time_step(dnlo *m, int force)
                 (m->timeStepComputationDate - m->lastTimeStepComputationDate);
     m->dt = ((dx) * (m->computedTimeStep) + (1.0 - (dx)) * (m->lastTimeStep)); /* LINEARdbl, BT/Math.h:91 */
   }
+  return O_OK;
 }
 
 /* boundary conditions on one node: BoundaryRestrain::applyConstant
@@ -1135,7 +1348,8 @@ int dnlo_begin(dnlo *m, double upTime)
   o_mass(m);
   st = dnlo_compute_internal_forces(m);
   if (st) return -st;
-  o_compute_time_step(m, 1);
+  st = o_compute_time_step(m, 1);
+  if (st) return -st;
   return 1;
 }
 
@@ -1159,7 +1373,8 @@ int dnlo_step(dnlo *m)
   {
     st = dnlo_compute_internal_matrices(m);
     if (st) return -st;
-    o_compute_time_step(m, 0);
+    st = o_compute_time_step(m, 0);
+    if (st) return -st;
   }
   return run;
 }
@@ -1233,6 +1448,7 @@ void dnlo_free(dnlo *m)
   free(m->x); free(m->x0);
   for (int f = 0; f < 2; f++) { free(m->v[f]); free(m->a[f]); free(m->du[f]); free(m->d[f]); }
   free(m->mass); free(m->bc); free(m->bcs); free(m->bcs_v); free(m->conn);
+  free(m->ev); free(m->Ke);
   free(m->S); free(m->Eps); free(m->EpsPlas); free(m->dEps); free(m->R); free(m->Sig);
   free(m->p); free(m->evp); free(m->T); free(m->dN); free(m->detJ); free(m->fint); free(m->Fe); free(m->Fe_status);
   free(m);

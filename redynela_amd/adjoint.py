@@ -47,6 +47,7 @@ class HardeningAdjoint:
         if self.dom is not None:
             self.dom.close()
         s = ExplicitSolverCH()
+        s.setTimeStepMethod("Courant")
         s.setDissipation(self.rho_b)
         self.dom = Domain(self.mesh, Material(self.E, self.nu, self.rho, IsoHardElastoplastic(A, B, n)), s)
         self.dom.set_option(_lib.OPT_STORE_FINT, 0)

@@ -73,9 +73,11 @@ static int snap_from(dnlg_handle *h, const Snap &L, const double *src) { return 
 
 static int adj_check(dnlg_handle *h)
 {
-  if (h->dim != DNLG_DIM_AXI || h->form != DNLG_FORM_FP64 || h->ea.law != 1 || h->has_bcs)
+  if (h->dim != DNLG_DIM_AXI || h->form != DNLG_FORM_FP64 || h->ea.law != 1 || h->has_bcs ||
+      h->ts_method != DNLG_TS_COURANT)
   {
-    h->err = "dnlg_adjoint: axisymmetric mesh, FP64 form, hardening A + B evp^n (law 1), no speed BC";
+    h->err = "dnlg_adjoint: axisymmetric mesh, FP64 form, hardening A + B evp^n (law 1), no speed BC, "
+             "Courant time step";
     return DNLG_E_ARG;
   }
   return DNLG_OK;

@@ -9,8 +9,12 @@
 //   each step : [Courant when the previous increment % freq == 0] k_dt
 //               (endStep of the previous increment, computeTimeStep, initStep)
 //               element kernel, fused nodal kernel
+// With the Pulse method a recomputing step splits k_dt around the power
+// iteration (endStep; Domain::getMaximumFrequency; computeTimeStep and
+// initStep), and the host waits for the iteration to converge.
 // and a final k_dt at every synchronisation point closes the last increment.
 #include "dnlg_handle.hpp"
+#include "dnlg_pulse.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -158,7 +162,8 @@ static void free_all(dnlg_handle *h)
   h->adj = nullptr;
   void *ptrs[] = {h->x, h->v, h->a, h->du, h->d, h->mass, h->fint, h->fe, h->S, h->evp, h->p, h->eps, h->epsplas, h->deps,
                   h->me, h->part, h->ke_part, h->bcs_v, h->conn, h->fpos, h->deg, h->bc, h->bcs, h->st, h->T,
-                  h->S32, h->evp32, h->T32, h->p32, h->fe32};
+                  h->S32, h->evp32, h->T32, h->p32, h->fe32, h->pulse_ke, h->pulse_val, h->pulse_init,
+                  h->pulse_part, h->pulse_ctl, h->pulse_y == h->fe ? nullptr : h->pulse_y};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -369,6 +374,9 @@ extern "C" int dnlg_destroy(dnlg_handle *h)
   return DNLG_OK;
 }
 
+static int finalize(dnlg_handle *h);
+static int read_state(dnlg_handle *h, StepState *s);
+
 extern "C" int dnlg_set_option(dnlg_handle *h, int option, int value)
 {
   if (!h) return DNLG_E_ARG;
@@ -396,6 +404,23 @@ extern "C" int dnlg_set_option(dnlg_handle *h, int option, int value)
     h->track_strain = value != 0;
     fill_elem_args(h);
     break;
+  case DNLG_OPT_TIME_STEP_METHOD:
+  {
+    if (value != DNLG_TS_COURANT && value != DNLG_TS_PULSE)
+    {
+      h->err = "setTimeStepMethod: Unknown method";
+      return DNLG_E_ARG;
+    }
+    int rc = finalize(h);
+    if (rc) return rc;
+    StepState s;
+    rc = read_state(h, &s);
+    if (rc) return rc;
+    s.ts_method = value;
+    DNLG_HIP(h, hipMemcpy(h->st, &s, sizeof s, hipMemcpyHostToDevice));
+    h->ts_method = value;
+    break;
+  }
   default: h->err = "dnlg_set_option: unknown option"; return DNLG_E_ARG;
   }
   return DNLG_OK;
@@ -415,10 +440,84 @@ static int check_status(dnlg_handle *h, const StepState &s)
   const char *what = s.status == ST_POLAR ? "Eigen Value Computation: No convergence (polar decomposition)"
                      : s.status == ST_RR  ? "Element::computeInternalForces: Too many atempts in plastic procedure"
                      : s.status == ST_DETJ ? "Negative or singular Jacobian"
+                     : s.status == ST_PULSE ? "Domain::getMaximumFrequency: power iteration method not converged"
                                            : "device error";
-  snprintf(buf, sizeof buf, "%s (element %d)", what, s.err_elem);
+  if (s.status == ST_PULSE)
+    snprintf(buf, sizeof buf, "%s", what);
+  else
+    snprintf(buf, sizeof buf, "%s (element %d)", what, s.err_elem);
   h->err = buf;
-  return s.status == ST_POLAR ? DNLG_E_POLAR_NOCONV : s.status == ST_RR ? DNLG_E_RR_NOCONV : DNLG_E_NEG_DETJ;
+  return s.status == ST_POLAR ? DNLG_E_POLAR_NOCONV : s.status == ST_RR ? DNLG_E_RR_NOCONV
+         : s.status == ST_PULSE ? DNLG_E_PULSE_NOCONV : DNLG_E_NEG_DETJ;
+}
+
+// Domain::getMaximumFrequency (FE/Domain.C:816-946): Ke of every element
+// once, then power iterations issued in growing batches, the host reading
+// the convergence flag between batches (the kernels of iterations issued
+// past convergence return at once).  Buffers are allocated at the first call.
+static int run_pulse(dnlg_handle *h)
+{
+  const int nd = h->nen * h->dim;
+  const long long ndof = h->nn * h->dim;
+  if (!h->pulse_ctl)
+  {
+    DNLG_HIP(h, dalloc(&h->pulse_ke, ((h->ne + 7) / 8) * 8 * (long long)nd * nd));
+    DNLG_HIP(h, dalloc(&h->pulse_val, ndof));
+    DNLG_HIP(h, dalloc(&h->pulse_init, ndof));
+    DNLG_HIP(h, dalloc(&h->pulse_part, nodal_blocks(h->nn)));
+    DNLG_HIP(h, dalloc(&h->pulse_ctl, 1));
+    if (h->fe)
+      h->pulse_y = h->fe;  // free between an increment's nodal pass and the next element pass
+    else
+      DNLG_HIP(h, dalloc(&h->pulse_y, (long long)h->maxdeg * h->nn * h->dim));
+    // start vector ev(0) = 1, ev(i) = ev(i-1) - 2/(noddl-1), FE/Domain.C:849-856
+    std::vector<double> ev(ndof);
+    ev[0] = 1.0;
+    for (long long i = 1; i < ndof; i++) ev[i] = ev[i - 1] - 2. / (ndof - 1);
+    DNLG_HIP(h, hipMemcpy(h->pulse_init, ev.data(), sizeof(double) * ndof, hipMemcpyHostToDevice));
+    PulseCtl c{};
+    c.invf = 1.0;
+    DNLG_HIP(h, hipMemcpy(h->pulse_ctl, &c, sizeof c, hipMemcpyHostToDevice));
+  }
+  PulseArgs p{};
+  p.conn = h->conn;
+  p.fpos = h->fpos;
+  p.deg = h->deg;
+  p.mass = h->mass;
+  p.ev_init = h->pulse_init;
+  p.ke = h->pulse_ke;
+  p.y = h->pulse_y;
+  p.val = h->pulse_val;
+  p.part = h->pulse_part;
+  p.ctl = h->pulse_ctl;
+  p.st = h->st;
+  p.ne = h->ne;
+  p.nn = h->nn;
+  p.ndof = ndof;
+  p.E = h->mat.young;
+  p.nu = h->mat.poisson;
+  launch_pulse_begin(p, h->stream);
+  launch_pulse_ke(h->dim, p, h->ea, h->stream);
+  int batch = 1;  // a call that starts from the last iterate often converges at once
+  for (;;)
+  {
+    for (int b = 0; b < batch; b++) launch_pulse_iteration(h->dim, p, h->stream);
+    DNLG_HIP(h, hipGetLastError());
+    PulseCtl c;
+    StepState s;
+    DNLG_HIP(h, hipMemcpyAsync(&c, h->pulse_ctl, sizeof c, hipMemcpyDeviceToHost, h->stream));
+    DNLG_HIP(h, hipMemcpyAsync(&s, h->st, sizeof s, hipMemcpyDeviceToHost, h->stream));
+    DNLG_HIP(h, hipStreamSynchronize(h->stream));
+    if (!s.running) return DNLG_OK;  // the solve loop ended: no computeTimeStep
+    if (c.done)
+    {
+      h->pulse_calls++;
+      h->pulse_iters += c.iter;
+      h->pulse_iters_last = c.iter;
+      return c.failed ? check_status(h, s) : DNLG_OK;
+    }
+    if (batch < 64) batch *= 2;
+  }
 }
 
 static void launch_elem(dnlg_handle *h, bool check_running, bool speed)
@@ -470,11 +569,22 @@ static int finalize(dnlg_handle *h)
 {
   if (!h->pending) return DNLG_OK;
   const int rec = (h->inc % h->freq) == 0;
-  if (rec) launch_courant_f64(h->dim, h->ea, h->part, h->stream);
-  launch_dt(h->st, h->part, (int)courant_blocks(h->ne), rec, h->ke_on ? h->ke_part : nullptr,
-            (int)nodal_blocks(h->nn), 1, h->stream);
-  DNLG_HIP(h, hipGetLastError());
   h->pending = false;
+  if (rec && h->ts_method == DNLG_TS_PULSE)
+  {
+    launch_dt(h->st, nullptr, 0, 0, h->ke_on ? h->ke_part : nullptr, (int)nodal_blocks(h->nn), 3, h->stream);
+    DNLG_HIP(h, hipGetLastError());
+    const int rc = run_pulse(h);
+    if (rc) return rc;
+    launch_dt(h->st, nullptr, 0, 1, nullptr, 0, 5, h->stream);
+  }
+  else
+  {
+    if (rec) launch_courant_f64(h->dim, h->ea, h->part, h->stream);
+    launch_dt(h->st, h->part, (int)courant_blocks(h->ne), rec, h->ke_on ? h->ke_part : nullptr,
+              (int)nodal_blocks(h->nn), 1, h->stream);
+  }
+  DNLG_HIP(h, hipGetLastError());
   return DNLG_OK;
 }
 
@@ -508,8 +618,18 @@ extern "C" int dnlg_begin(dnlg_handle *h, double up_time, int *running)
   launch_gather_fint(h->dim, h->deg, h->fe, h->form == DNLG_FORM_MIXED ? h->fe32 : nullptr, h->fint, h->nn,
                      h->stream);
   // computeTimeStep(True) (FE/ExplicitSolverCH.C:172)
-  launch_courant_f64(h->dim, h->ea, h->part, h->stream);
-  launch_dt(h->st, h->part, (int)courant_blocks(h->ne), 1, nullptr, 0, 2, h->stream);
+  if (h->ts_method == DNLG_TS_PULSE)
+  {
+    DNLG_HIP(h, hipGetLastError());
+    rc = run_pulse(h);
+    if (rc) return rc;
+    launch_dt(h->st, nullptr, 0, 1, nullptr, 0, 2, h->stream);
+  }
+  else
+  {
+    launch_courant_f64(h->dim, h->ea, h->part, h->stream);
+    launch_dt(h->st, h->part, (int)courant_blocks(h->ne), 1, nullptr, 0, 2, h->stream);
+  }
   DNLG_HIP(h, hipGetLastError());
   rc = read_state(h, &s);
   if (rc) return rc;
@@ -528,12 +648,21 @@ static int issue_steps(dnlg_handle *h, int nsteps)
   for (int s = 0; s < nsteps; s++)
   {
     int rec = 0;
-    if (h->pending)
+    if (h->pending) rec = (h->inc % h->freq) == 0;
+    if (rec && h->ts_method == DNLG_TS_PULSE)
     {
-      rec = (h->inc % h->freq) == 0;
-      if (rec) launch_courant_f64(h->dim, h->ea, h->part, h->stream);
+      // endStep; getMaximumFrequency (host waits); computeTimeStep + initStep
+      launch_dt(h->st, nullptr, 0, 0, h->ke_on ? h->ke_part : nullptr, nke, 3, h->stream);
+      DNLG_HIP(h, hipGetLastError());
+      const int rc = run_pulse(h);
+      if (rc) return rc;
+      launch_dt(h->st, nullptr, 0, 1, nullptr, 0, 4, h->stream);
     }
-    launch_dt(h->st, h->part, npart, rec, h->ke_on ? h->ke_part : nullptr, nke, 0, h->stream);
+    else
+    {
+      if (rec) launch_courant_f64(h->dim, h->ea, h->part, h->stream);
+      launch_dt(h->st, h->part, npart, rec, h->ke_on ? h->ke_part : nullptr, nke, 0, h->stream);
+    }
     if (h->timing)
     {
       hipEvent_t e0 = next_event(h), e1 = next_event(h), e2 = next_event(h), e3 = next_event(h);
@@ -597,6 +726,25 @@ extern "C" int dnlg_step(dnlg_handle *h, int nsteps, int *running)
   if (running) *running = s.running;
   if (!s.running) h->begun = false;
   return check_status(h, s);
+}
+
+extern "C" int dnlg_pulse_info(dnlg_handle *h, double *out4)
+{
+  if (!h || !out4) return DNLG_E_ARG;
+  double fmax0 = 0.0, conv = 0.0;
+  if (h->pulse_ctl)
+  {
+    PulseCtl c;
+    DNLG_HIP(h, hipMemcpyAsync(&c, h->pulse_ctl, sizeof c, hipMemcpyDeviceToHost, h->stream));
+    DNLG_HIP(h, hipStreamSynchronize(h->stream));
+    fmax0 = c.fmax0;
+    conv = c.conv;
+  }
+  out4[0] = (double)h->pulse_calls;
+  out4[1] = (double)h->pulse_iters_last;
+  out4[2] = fmax0;
+  out4[3] = conv;
+  return DNLG_OK;
 }
 
 extern "C" int dnlg_solve(dnlg_handle *h, double up_time, long long *steps_done)

@@ -664,14 +664,15 @@ __device__ __forceinline__ double quad_length(const double *x, const int *cn)
   return ((x31 * y42 + x24 * y31) / sqrt(2. * ((x24 * x24) + (y42 * y42) + (x31 * x31) + (y31 * y31))));
 }
 
-// Solver::computeTimeStep (FE/Solver.C:220-267) given the Courant minimum when
-// it recomputes; ExplicitSolverCH::initStep (FE/ExplicitSolverCH.C:434-447)
+// Solver::computeTimeStep (FE/Solver.C:220-267) given the Courant minimum
+// (or, for the Pulse method, the frequency left in pulse_f) when it
+// recomputes; ExplicitSolverCH::initStep (FE/ExplicitSolverCH.C:434-447)
 __device__ __forceinline__ void compute_time_step(StepState *st, int recompute, double crit)
 {
 #pragma clang fp contract(off)
   if (recompute)
   {
-    const double fmax = 2.0 / crit;
+    const double fmax = st->ts_method == 1 ? st->pulse_f : 2.0 / crit;
     st->dt = st->safety * st->omegaS / fmax;
     st->lastTsDate = st->tsDate;
     st->lastTimeStep = st->computedTimeStep;

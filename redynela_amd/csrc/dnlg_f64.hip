@@ -258,6 +258,11 @@ __global__ __launch_bounds__(256) void k_courant(ElemArgs a, double *part)
 //           then ExplicitSolverCH::initStep of the new increment;
 //   mode 1 (sync point): only the pending end of the last increment;
 //   mode 2 (solve() preamble): forced Solver::computeTimeStep.
+// For the Pulse method a recomputing step splits in two around the power
+// iteration (its frequency is then in st->pulse_f):
+//   mode 3: the pending endStep only;
+//   mode 4: forced computeTimeStep then initStep, while running;
+//   mode 5: forced computeTimeStep only, while running.
 // FE/ExplicitSolverCH.C:172, 244-270, 434-468; FE/Solver.C:169-176, 220-267.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_dt(StepState *st, const double *part, int npart, int recompute,
@@ -265,10 +270,11 @@ __global__ __launch_bounds__(256) void k_dt(StepState *st, const double *part, i
 {
   __shared__ double red[256];
   __shared__ int go, do_ts, run0, pend0;
+  const bool forced = mode == 2 || mode == 4 || mode == 5;
   if (threadIdx.x == 0)
   {
     run0 = st->running;
-    pend0 = (mode != 2) && st->pending;
+    pend0 = !forced && st->pending;
   }
   __syncthreads();
   if (!run0) return;
@@ -297,12 +303,12 @@ __global__ __launch_bounds__(256) void k_dt(StepState *st, const double *part, i
       go = end_step(st);
       do_ts = go;
     }
-    if (mode == 2) do_ts = 1;
+    if (forced) do_ts = 1;
   }
   __syncthreads();
-  if (!go) return;
+  if (!go || mode == 3) return;
   double crit = __builtin_inf();
-  const int rec = do_ts && (recompute || mode == 2);
+  const int rec = do_ts && (recompute || forced);
   if (rec)
   {
     for (int k = threadIdx.x; k < npart; k += 256) crit = fmin(crit, part[k]);
@@ -317,7 +323,7 @@ __global__ __launch_bounds__(256) void k_dt(StepState *st, const double *part, i
   }
   if (threadIdx.x != 0) return;
   if (do_ts) compute_time_step(st, rec, crit);
-  if (mode == 0) init_step(st);
+  if (mode == 0 || mode == 4) init_step(st);
 }
 
 // ---------------------------------------------------------------------------

@@ -9,6 +9,7 @@
 
 namespace dnl {
 struct AdjStore;  // checkpoints and tape of the adjoint (dnlg_adjoint_api.cpp)
+struct PulseCtl;  // power-iteration state (dnlg_pulse.hpp)
 }  // namespace dnl
 using namespace dnl;
 
@@ -48,6 +49,12 @@ struct dnlg_handle
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   AdjStore *adj = nullptr;  // checkpoints of dnlg_adjoint_forward, freed by dnlg_destroy
+  // Pulse time step (Domain::getMaximumFrequency), allocated at its first use
+  int ts_method = 0;
+  double *pulse_ke = nullptr, *pulse_y = nullptr, *pulse_val = nullptr, *pulse_init = nullptr,
+         *pulse_part = nullptr;
+  PulseCtl *pulse_ctl = nullptr;
+  long long pulse_calls = 0, pulse_iters = 0, pulse_iters_last = 0;
   std::string err;
 };
 

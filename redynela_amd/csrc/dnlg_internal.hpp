@@ -7,7 +7,7 @@
 namespace dnl {
 
 // device status codes (mirror include/dnlg.h)
-enum : int { ST_OK = 0, ST_POLAR = 2, ST_RR = 3, ST_DETJ = 4, ST_BUDGET = 6 };
+enum : int { ST_OK = 0, ST_POLAR = 2, ST_RR = 3, ST_DETJ = 4, ST_BUDGET = 6, ST_PULSE = 7 };
 
 // Time-integration scalars kept on the device so that a step is launch-only.
 // Mirrors Solver / ExplicitSolverCH / Domain scalar state:
@@ -29,6 +29,9 @@ struct StepState
   int err_elem;
   int pending;       // the last issued increment still needs its endStep
   int active;        // batched stepped engine: this increment runs its nodal phase
+  int ts_method;     // Solver::timeStepMethod: 0 Courant, 1 Pulse
+  int pad2;
+  double pulse_f;    // Domain::getMaximumFrequency of the last Pulse computation
 };
 
 struct ElemArgs

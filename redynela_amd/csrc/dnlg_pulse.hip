@@ -1,0 +1,392 @@
+// Pulse time step on the GPU: Domain::getMaximumFrequency (FE/Domain.C:816-946).
+//
+// The reference forms every element's elastic stiffness matrix
+// (FE/Element3D.C:242-333 for hex8, FE/Element2D.C:129-222 for the
+// axisymmetric quad) from the point dN and detJ of the last
+// computeInternalMatrices, then runs a power iteration on M^-1 K: each
+// iteration scatters the iterate to the elements, multiplies by Ke with
+// LAPACK's reference dgemv, assembles element by element into a zeroed
+// vector, divides by the lumped mass, takes the largest magnitude fmax and
+// scales the iterate by 1/fmax, until |fmax - fmax0| / fmax < 1e-4.  The
+// iterate and fmax0 carry over from one call to the next.
+//
+// Kernels (one call = pulse_ke once, then pulse_mv, pulse_node and
+// pulse_ctl per iteration; all of them return at once when the loop has
+// stopped or converged, so the host may issue iterations ahead):
+//   k_pulse_begin  reset the iteration counters; start vector when fmax0 == 0
+//   k_pulse_ke     Ke of 8 elements per block, one thread per row; per point
+//                  the rows share dN and the columns of C B through LDS.
+//                  Ke is stored [block][column J][8 elements x ND rows] so
+//                  that a row's entries for one J are adjacent across threads
+//   k_pulse_mv     y = Ke x for one row per thread (dgemv order: y = 0, then
+//                  y += x_J Ke(I, J) for J in order), y into the element's
+//                  ELL slot of the node
+//   k_pulse_node   per node: slot sum in element order (the reference's
+//                  assembly order), divide by the mass, block max of |val|
+//   k_pulse_ctl    max over blocks, 1/fmax, convergence, iteration limit
+// The stored iterate is val (before the 1/fmax scaling) and invf = 1/fmax;
+// the matvec reads val * invf, which is the reference's ev(i) *= 1/fmax.
+#include "dnlg_elem_f64.hpp"
+#include "dnlg_pulse.hpp"
+
+namespace dnl {
+
+// Material::getHookeMatrix (FE/Material.C:270-331), row-major: the
+// threedimensional 6x6 (DIM 3) or the axisymetric 4x4 (DIM 2)
+template <int DIM>
+__device__ void hooke(double E, double Nu, double *C)
+{
+#pragma clang fp contract(off)
+  constexpr int NC = DIM == 3 ? 6 : 4;
+  for (int k = 0; k < NC * NC; k++) C[k] = 0.0;
+  if constexpr (DIM == 3)
+  {
+    C[0] = 1; C[7] = 1; C[14] = 1;
+    C[21] = (1.0 - 2.0 * Nu) / (2. * (1.0 - Nu));
+    C[28] = (1.0 - 2.0 * Nu) / (2. * (1.0 - Nu));
+    C[35] = (1.0 - 2.0 * Nu) / (2. * (1.0 - Nu));
+    C[1] = C[6] = Nu / (1. - Nu);
+    C[2] = C[12] = Nu / (1. - Nu);
+    C[8] = C[13] = Nu / (1. - Nu);
+  }
+  else
+  {
+    C[0] = 1; C[5] = 1; C[15] = 1;
+    C[4] = C[1] = Nu / (1. - Nu);
+    C[12] = C[13] = Nu / (1. - Nu);
+    C[3] = C[7] = Nu / (1. - Nu);
+    C[10] = (1.0 - 2.0 * Nu) / (2. * (1.0 - Nu));
+  }
+  const double f = (E * (1.0 - Nu) / ((1.0 + Nu) * (1.0 - 2.0 * Nu)));
+  for (int k = 0; k < NC * NC; k++) C[k] *= f;
+}
+
+__global__ __launch_bounds__(256) void k_pulse_begin(PulseArgs p)
+{
+  if (!p.st->running) return;
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  const bool fresh = p.ctl->fmax0 == 0.0;  // FE/Domain.C:849-856
+  if (fresh && i < p.ndof) p.val[i] = p.ev_init[i];
+  if (i == 0)
+  {
+    if (fresh) p.ctl->invf = 1.0;
+    p.ctl->iter = 0;
+    p.ctl->done = 0;
+  }
+}
+
+template <int NEN, int DIM>
+struct PulseLds
+{
+  static constexpr int ND = NEN * DIM, NIP = NEN, NC = DIM == 3 ? 6 : 4;
+  double tab[NIP * NEN * 3];     // dN0 (and N for the quad) of the reference element
+  double C[NC * NC];
+  double geo[8][NIP][DIM * DIM + 2];  // inv(J), W (with 2 pi R for the quad), R
+  double dN[8][NEN][DIM];
+  double CB[8][NC][ND];
+};
+
+template <int NEN, int DIM>
+__global__ __launch_bounds__(8 * NEN * DIM) void k_pulse_ke(PulseArgs p, ElemArgs a)
+{
+#pragma clang fp contract(off)
+  using L = PulseLds<NEN, DIM>;
+  constexpr int ND = L::ND, NIP = L::NIP, NT = 8 * ND;
+  __shared__ L s;
+  if (!p.st->running) return;
+  const int t = threadIdx.x, el = t / ND, I = t % ND, i = I / DIM, comp = I % DIM;
+  const long long e_raw = (long long)blockIdx.x * 8 + el;
+  const bool valid = e_raw < a.ne;
+  const long long e = valid ? e_raw : a.ne - 1;
+  for (int k = t; k < NIP * NEN * 3; k += NT)
+  {
+    const int pt = k / (NEN * 3), n = (k / 3) % NEN, c = k % 3;
+    if constexpr (DIM == 3)
+    {
+      double g[3];
+      hex_dn0(a, pt, n, g);
+      s.tab[k] = g[c];
+    }
+    else
+    {
+      double g[2];
+      quad_dn0(a, pt, n, g);
+      s.tab[k] = c < 2 ? g[c] : quad_n(a, pt, n);
+    }
+  }
+  if (t == 0) hooke<DIM>(p.E, p.nu, s.C);
+  __syncthreads();
+  const int *cn = a.conn + e * NEN;
+  // point geometry: FE/Element3D.C:75-105, FE/Element2D.C:84-115; thread I < NIP takes point I
+  if (I < NIP)
+  {
+    const int pt = I;
+    double *gq = s.geo[el][pt];
+    if constexpr (DIM == 3)
+    {
+      T9 J = {{0, 0, 0, 0, 0, 0, 0, 0, 0}};
+      for (int k = 0; k < 8; k++)
+      {
+        const double *xk = a.x + 3 * (long long)cn[k];
+        const double x0 = xk[0], x1 = xk[1], x2 = xk[2];
+        const double *g = s.tab + (pt * NEN + k) * 3;
+        J.v[0] += g[0] * x0; J.v[1] += g[1] * x0; J.v[2] += g[2] * x0;
+        J.v[3] += g[0] * x1; J.v[4] += g[1] * x1; J.v[5] += g[2] * x1;
+        J.v[6] += g[0] * x2; J.v[7] += g[1] * x2; J.v[8] += g[2] * x2;
+      }
+      const double detJ = (J.v[0] * J.v[4] * J.v[8]) + (J.v[3] * J.v[7] * J.v[2]) + (J.v[6] * J.v[1] * J.v[5]) -
+                          (J.v[6] * J.v[4] * J.v[2]) - (J.v[0] * J.v[7] * J.v[5]) - (J.v[3] * J.v[1] * J.v[8]);
+      T9 inv;
+      inv3(J, inv);  // a singular J was reported by the element pass that precedes every call
+      for (int c = 0; c < 9; c++) gq[c] = inv.v[c];
+      gq[9] = 1.0 * detJ;
+      gq[10] = 0.0;
+    }
+    else
+    {
+      double J0 = 0, J1 = 0, J2 = 0, J3 = 0, R = 0.;
+      for (int k = 0; k < 4; k++)
+      {
+        const double *xk = a.x + 3 * (long long)cn[k];
+        const double x0 = xk[0], x1 = xk[1];
+        const double *g = s.tab + (pt * NEN + k) * 3;
+        J0 += g[0] * x0; J1 += g[1] * x0;
+        J2 += g[0] * x1; J3 += g[1] * x1;
+      }
+      for (int k = 0; k < 4; k++) R += s.tab[(pt * NEN + k) * 3 + 2] * a.x[3 * (long long)cn[k]];
+      const double detJ = (J0 * J3) - (J1 * J2);
+      gq[0] = J3 / detJ; gq[1] = -J1 / detJ; gq[2] = -J2 / detJ; gq[3] = J0 / detJ;
+      double W = 1.0 * detJ;
+      W *= kDeuxPI * R;  // 2 * PI * R: doubling is exact
+      gq[4] = W;
+      gq[5] = R;
+    }
+  }
+  double k[ND];
+#pragma unroll
+  for (int J = 0; J < ND; J++) k[J] = 0.0;
+  const double *C = s.C;
+  for (int pt = 0; pt < NIP; pt++)
+  {
+    __syncthreads();  // geo written / previous point's dN and CB consumed
+    const double *gq = s.geo[el][pt];
+    if (comp == 0)
+    {
+      const double *g = s.tab + (pt * NEN + i) * 3;
+      double d[DIM];
+      dgemm_row<DIM>(g, gq, d);
+#pragma unroll
+      for (int c = 0; c < DIM; c++) s.dN[el][i][c] = d[c];
+    }
+    __syncthreads();
+    const double W = gq[DIM * DIM];
+    // column I of C B
+    {
+      const double *d = s.dN[el][i];
+      if constexpr (DIM == 3)
+      {
+        const double d0 = d[0], d1 = d[1], d2 = d[2];
+#pragma unroll
+        for (int r = 0; r < 6; r++)
+        {
+          double v;
+          if (comp == 0) v = (C[r * 6 + 0] * d0 + C[r * 6 + 3] * d1 + C[r * 6 + 5] * d2);
+          else if (comp == 1) v = (C[r * 6 + 1] * d1 + C[r * 6 + 3] * d0 + C[r * 6 + 4] * d2);
+          else v = (C[r * 6 + 2] * d2 + C[r * 6 + 4] * d1 + C[r * 6 + 5] * d0);
+          s.CB[el][r][I] = v;
+        }
+      }
+      else
+      {
+        const double d0 = d[0], d1 = d[1], R = gq[5], Ni = s.tab[(pt * NEN + i) * 3 + 2];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+        {
+          double v;
+          if (comp == 0)
+          {
+            v = (C[r * 4 + 0] * d0 + C[r * 4 + 2] * d1);
+            v += C[r * 4 + 3] * Ni / R;
+          }
+          else
+            v = (C[r * 4 + 1] * d1 + C[r * 4 + 2] * d0);
+          s.CB[el][r][I] = v;
+        }
+      }
+    }
+    __syncthreads();
+    // row I of B^T C B
+    {
+      const double *d = s.dN[el][i];
+      if constexpr (DIM == 3)
+      {
+        const double a0 = d[0], a1 = d[1], a2 = d[2];
+        const int r0 = comp == 0 ? 0 : (comp == 1 ? 1 : 2);
+        const int r1 = comp == 0 ? 3 : (comp == 1 ? 3 : 4);
+        const int r2 = comp == 0 ? 5 : (comp == 1 ? 4 : 5);
+        const double b0 = comp == 0 ? a0 : (comp == 1 ? a1 : a2);
+        const double b1 = comp == 0 ? a1 : (comp == 1 ? a0 : a1);
+        const double b2 = comp == 0 ? a2 : (comp == 1 ? a2 : a0);
+#pragma unroll
+        for (int J = 0; J < ND; J++)
+          k[J] += (b0 * s.CB[el][r0][J] + b1 * s.CB[el][r1][J] + b2 * s.CB[el][r2][J]) * W;
+      }
+      else
+      {
+        const double a0 = d[0], a1 = d[1];
+        if (comp == 0)
+        {
+          const double R = gq[5], Ni = s.tab[(pt * NEN + i) * 3 + 2];
+#pragma unroll
+          for (int J = 0; J < ND; J++)
+          {
+            k[J] += (a0 * s.CB[el][0][J] + a1 * s.CB[el][2][J]) * W;
+            k[J] += s.CB[el][3][J] * Ni / R * W;
+          }
+        }
+        else
+        {
+#pragma unroll
+          for (int J = 0; J < ND; J++) k[J] += (a1 * s.CB[el][1][J] + a0 * s.CB[el][2][J]) * W;
+        }
+      }
+    }
+  }
+  if (!valid) return;
+  double *ko = p.ke + (long long)blockIdx.x * ND * NT + t;
+#pragma unroll
+  for (int J = 0; J < ND; J++) ko[(long long)J * NT] = k[J];
+}
+
+template <int NEN, int DIM>
+__global__ __launch_bounds__(8 * NEN * DIM) void k_pulse_mv(PulseArgs p)
+{
+#pragma clang fp contract(off)
+  constexpr int ND = NEN * DIM, NT = 8 * ND;
+  __shared__ double loc[8][ND];
+  __shared__ double s_invf;
+  if (!p.st->running || p.ctl->done) return;
+  const int t = threadIdx.x, el = t / ND, I = t % ND;
+  const long long e_raw = (long long)blockIdx.x * 8 + el;
+  const bool valid = e_raw < p.ne;
+  const long long e = valid ? e_raw : p.ne - 1;
+  if (t == 0) s_invf = p.ctl->invf;
+  __syncthreads();
+  const int node = p.conn[e * NEN + I / DIM];
+  loc[el][I] = p.val[(long long)node * DIM + I % DIM] * s_invf;  // ev(i) *= 1/fmax, LA/Vector.C:652-659
+  __syncthreads();
+  const double *kr = p.ke + (long long)blockIdx.x * ND * NT + t;
+  double y = 0.0;
+#pragma unroll
+  for (int J = 0; J < ND; J++)
+  {
+    const double tj = 1.0 * loc[el][J];
+    y = y + tj * kr[(long long)J * NT];
+  }
+  if (valid) p.y[(long long)p.fpos[e * NEN + I / DIM] * DIM + I % DIM] = y;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(256) void k_pulse_node(PulseArgs p)
+{
+#pragma clang fp contract(off)
+  __shared__ double red[256];
+  if (!p.st->running || p.ctl->done) return;
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  double mx = 0.0;
+  if (i < p.nn)
+  {
+    const int d = p.deg[i];
+    const double m = p.mass[i];
+#pragma unroll
+    for (int j = 0; j < DIM; j++)
+    {
+      double acc = 0.0;
+      for (int k = 0; k < d; k++) acc += p.y[((long long)k * p.nn + i) * DIM + j];
+      double val = 0.0;
+      val += acc;
+      val /= m;
+      if (fabs(val) > mx) mx = fabs(val);
+      p.val[i * DIM + j] = val;
+    }
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1)
+  {
+    if (threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) p.part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_pulse_ctl(PulseArgs p, int nblk)
+{
+#pragma clang fp contract(off)
+  __shared__ double red[256];
+  if (!p.st->running || p.ctl->done) return;
+  double mx = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += 256)
+    if (p.part[k] > mx) mx = p.part[k];
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1)
+  {
+    if (threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  PulseCtl *c = p.ctl;
+  const double fmax = red[0];
+  const int it = ++c->iter;
+  c->invf = 1.0 / fmax;
+  const double conv = fabs(fmax - c->fmax0) / fmax;
+  c->conv = conv;
+  c->fmax = fmax;
+  c->fmax0 = fmax;
+  if (it > PULSE_MAX_ITER)  // FE/Domain.C:936-939, checked after the convergence test
+  {
+    c->done = 1;
+    c->failed = 1;
+    raise_status(p.st, ST_PULSE, -1);
+  }
+  else if (conv < PULSE_CONVERGENCE)
+  {
+    c->done = 1;
+    p.st->pulse_f = sqrt(fmax);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+void launch_pulse_begin(const PulseArgs &p, hipStream_t s)
+{
+  const unsigned grid = (unsigned)((p.ndof + 255) / 256);
+  hipLaunchKernelGGL(k_pulse_begin, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, p);
+}
+
+void launch_pulse_ke(int dim, const PulseArgs &p, const ElemArgs &a, hipStream_t s)
+{
+  const unsigned grid = (unsigned)((a.ne + 7) / 8);
+  if (dim == 3) hipLaunchKernelGGL((k_pulse_ke<8, 3>), dim3(grid), dim3(192), 0, s, p, a);
+  else hipLaunchKernelGGL((k_pulse_ke<4, 2>), dim3(grid), dim3(64), 0, s, p, a);
+}
+
+void launch_pulse_iteration(int dim, const PulseArgs &p, hipStream_t s)
+{
+  const unsigned grid_e = (unsigned)((p.ne + 7) / 8), grid_n = (unsigned)((p.nn + 255) / 256);
+  if (dim == 3)
+  {
+    hipLaunchKernelGGL((k_pulse_mv<8, 3>), dim3(grid_e), dim3(192), 0, s, p);
+    hipLaunchKernelGGL((k_pulse_node<3>), dim3(grid_n), dim3(256), 0, s, p);
+  }
+  else
+  {
+    hipLaunchKernelGGL((k_pulse_mv<4, 2>), dim3(grid_e), dim3(64), 0, s, p);
+    hipLaunchKernelGGL((k_pulse_node<2>), dim3(grid_n), dim3(256), 0, s, p);
+  }
+  hipLaunchKernelGGL(k_pulse_ctl, dim3(1), dim3(256), 0, s, p, (int)grid_n);
+}
+
+}  // namespace dnl

@@ -388,13 +388,15 @@ class SolverObj(Obj):
     def __init__(self):
         self.start, self.stop = 0.0, 1e300
         self.rho_b, self.safety, self.freq = 0.8182, 0.9, 10
+        self.method = "Pulse"  # FE/ExplicitSolverCH.C:83
 
     def m_setTimes(self, deck, a, b):
         self.start, self.stop = float(a), float(b)
 
-    def m_setTimeStepMethod(self, deck, name):
-        if name != "Courant":
-            raise DeckError("ExplicitSolverCH::setTimeStepMethod: only Courant is implemented on the GPU path")
+    def m_setTimeStepMethod(self, deck, name):  # FE/Solver.C:312-327
+        if name not in ("Courant", "Pulse"):
+            raise DeckError("setTimeStepMethod: Unknown method %s" % name)
+        self.method = name
 
     def m_setDissipation(self, deck, v):
         if v < 0.0 or v > 1.0:
@@ -666,6 +668,7 @@ class Deck:
         hard = D.IsoHardElastoplastic(h.A, h.B, h.n) if h is not None else None
         m = D.Material(mat.young, mat.poisson, mat.density, hard)
         s = D.ExplicitSolverCH()
+        s.setTimeStepMethod(sol.method)
         s.setDissipation(sol.rho_b)
         s.setTimeStepSafetyFactor(sol.safety)
         s.setComputeTimeStepFrequency(sol.freq)

@@ -85,8 +85,9 @@ class Material:
 
 class ExplicitSolverCH:
     """Chung-Hulbert explicit solver parameters; default spectral radius 0.8182
-    (FE/DefaultValues.h:25), safety factor 0.9, Courant recompute every 10
-    increments (FE/DefaultValues.h:24)."""
+    (FE/DefaultValues.h:25), safety factor 0.9, time step recomputed every 10
+    increments (FE/DefaultValues.h:24) by the Pulse method, the reference's
+    default for this solver (FE/ExplicitSolverCH.C:83)."""
 
     def __init__(self):
         self.rho_b = 0.8182
@@ -94,6 +95,7 @@ class ExplicitSolverCH:
         self.freq = 10
         self.start_time, self.end_time = 0.0, 1e300
         self.form = _lib.FORM_FP64
+        self.method = "Pulse"
         self.domain = None
 
     def setPrecision(self, form):
@@ -115,8 +117,13 @@ class ExplicitSolverCH:
         self.freq = int(n)
 
     def setTimeStepMethod(self, name):
-        if name != "Courant":
-            raise DnlgError("only the Courant time-step method is implemented on the GPU path")
+        """'Courant' (characteristic length) or 'Pulse' (power iteration on
+        M^-1 K), Solver::setTimeStepMethod (FE/Solver.C:312-327)."""
+        if name not in ("Courant", "Pulse"):
+            raise DnlgError("setTimeStepMethod: Unknown method %s" % name)
+        self.method = name
+        if self.domain is not None:
+            self.domain.set_option(_lib.OPT_TIME_STEP_METHOD, _lib.TS_PULSE if name == "Pulse" else _lib.TS_COURANT)
 
     def solve(self, upTime):
         return self.domain.solve(upTime)
@@ -154,6 +161,8 @@ class Domain:
         self.h = h
         if track_strain:
             self.set_option(_lib.OPT_TRACK_STRAIN, 1)
+        if solver.method == "Pulse":
+            self.set_option(_lib.OPT_TIME_STEP_METHOD, _lib.TS_PULSE)
 
     # -- reference interface -------------------------------------------------
     def computeInternalForces(self):
@@ -219,6 +228,12 @@ class Domain:
         _lib.check(_lib.lib().dnlg_get_time(self.h, out.ctypes.data), self.h)
         return dict(t=out[0], dt=out[1], increment=int(out[2]), computed_dt=out[3], ke=out[4],
                     alphaM=out[5], beta=out[6], gamma=out[7], omegaS=out[8])
+
+    def pulse_info(self):
+        """Pulse time step: calls, iterations of the last call, fmax0 and its convergence measure."""
+        out = np.zeros(4)
+        _lib.check(_lib.lib().dnlg_pulse_info(self.h, out.ctypes.data), self.h)
+        return dict(calls=int(out[0]), iterations=int(out[1]), fmax0=out[2], convergence=out[3])
 
     def enable_kernel_timing(self, on=True):
         _lib.check(_lib.lib().dnlg_enable_kernel_timing(self.h, 1 if on else 0), self.h)

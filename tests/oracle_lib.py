@@ -70,6 +70,17 @@ def lib():
         L.dnlo_courant_crit.argtypes = [C.c_void_p]
         L.dnlo_element_length.restype = C.c_double
         L.dnlo_element_length.argtypes = [C.c_void_p, C.c_long]
+        L.dnlo_hooke.restype = C.c_int
+        L.dnlo_hooke.argtypes = [C.c_void_p, _dp]
+        L.dnlo_element_stiffness.argtypes = [C.c_void_p, C.c_long, _dp]
+        L.dnlo_max_frequency.restype = C.c_int
+        L.dnlo_max_frequency.argtypes = [C.c_void_p, C.c_void_p]
+        L.dnlo_set_ts_method.argtypes = [C.c_void_p, C.c_int]
+        L.dnlo_compute_mass.argtypes = [C.c_void_p]
+        L.dnlo_pulse_iterations.restype = C.c_long
+        L.dnlo_pulse_iterations.argtypes = [C.c_void_p]
+        L.dnlo_pulse_fmax0.restype = C.c_double
+        L.dnlo_pulse_fmax0.argtypes = [C.c_void_p]
         L.dnlo_get.restype = C.c_long
         L.dnlo_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.dnlo_get_time.argtypes = [C.c_void_p, _dp]
@@ -107,6 +118,8 @@ class Model:
         L.dnlo_set_solver(self.h, solver.get("rho_b", 0.8182), solver.get("safety", 0.9),
                           int(solver.get("freq", 10)), solver.get("end_time", 1e300))
         L.dnlo_set_mathmode(self.h, mathmode)
+        if solver.get("ts_method", "Courant") == "Pulse":  # Solver::setTimeStepMethod
+            L.dnlo_set_ts_method(self.h, 1)
         if "rr_prec" in solver:  # test-only: tighter radial-return stop than the reference's 1e-8
             L.dnlo_set_rr_precision(self.h, solver["rr_prec"])
         if "jc" in mat:  # Johnson-Cook rate / thermal extension
@@ -145,6 +158,29 @@ class Model:
     def solve(self, up_time):
         return lib().dnlo_solve(self.h, up_time)
 
+    def hooke(self):
+        C_ = np.zeros(36)
+        nc = lib().dnlo_hooke(self.h, C_)
+        return C_[:nc * nc].reshape(nc, nc)
+
+    def element_stiffness(self, e):
+        nd = (8 * 3) if self.dim == 3 else (4 * 2)
+        K = np.zeros(nd * nd)
+        lib().dnlo_element_stiffness(self.h, e, K)
+        return K.reshape(nd, nd)
+
+    def max_frequency(self):
+        """Domain::getMaximumFrequency: (sqrt(fmax), iterations), raises on no convergence"""
+        lib().dnlo_compute_mass(self.h)
+        out = C.c_double()
+        st = lib().dnlo_max_frequency(self.h, C.byref(out))
+        if st:
+            raise RuntimeError("oracle: power iteration status %d" % st)
+        return out.value, lib().dnlo_pulse_iterations(self.h)
+
+    def compute_internal_matrices(self):
+        return lib().dnlo_compute_internal_matrices(self.h)
+
     def compute_internal_forces(self):
         return lib().dnlo_compute_internal_forces(self.h)
 
@@ -163,7 +199,7 @@ def from_deck(d, mathmode=1):
     h = mat.hardening
     m = dict(E=mat.young, nu=mat.poisson, rho=mat.density, A=h.A if h else 0.0, B=h.B if h else 0.0,
              n=h.n if h else 1.0, **({} if h else {"yield": 0}))
-    s = dict(rho_b=sol.rho_b, safety=sol.safety, freq=sol.freq, end_time=sol.stop)
+    s = dict(rho_b=sol.rho_b, safety=sol.safety, freq=sol.freq, end_time=sol.stop, ts_method=sol.method)
     return Model(mesh, m, s, mathmode)
 
 

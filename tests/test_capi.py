@@ -52,6 +52,7 @@ def test_create_rejects_bad_arguments_without_touching_the_gpu():
 def test_set_dissipation_range_like_the_reference():
     import redynela_amd as R
     s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
     with pytest.raises(R.DnlgError):
         s.setDissipation(1.2)
     s.setDissipation(1.0)

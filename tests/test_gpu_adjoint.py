@@ -64,6 +64,7 @@ def test_adjoint_forward_is_the_production_stepping():
     p = HardeningAdjoint(mesh, nsteps=150, ckpt_every=40)
     x = p.forward(th)
     s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
     s.setDissipation(1.0)
     d = R.Domain(mesh, R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(*th)), s)
     d.begin(1e300)

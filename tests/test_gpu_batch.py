@@ -29,6 +29,7 @@ def say(*a):
 def solver(form="fp64"):
     import redynela_amd as R
     s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
     s.setDissipation(1.0)
     s.setPrecision(form)
     return s

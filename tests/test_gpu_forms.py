@@ -38,6 +38,7 @@ def gpu_domain(mesh, mat, form="fp64", rho_b=1.0):
         h = R.IsoHardElastoplastic(mat["A"], mat["B"], mat["n"])
     m = R.Material(mat["E"], mat["nu"], mat["rho"], h)
     s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
     s.setDissipation(rho_b)
     s.setPrecision(form)
     return R.Domain(mesh, m, s)

@@ -25,6 +25,7 @@ def gpu_domain(mesh):
     m = R.Material(COPPER["E"], COPPER["nu"], COPPER["rho"], R.IsoHardElastoplastic(COPPER["A"], COPPER["B"],
                                                                                    COPPER["n"]))
     s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
     s.setDissipation(1.0)
     return R.Domain(mesh, m, s)
 

@@ -30,6 +30,7 @@ def make_pair(mesh, mat, rho_b=1.0, track=False):
     import redynela_amd as R
     m = R.Material(mat["E"], mat["nu"], mat["rho"], R.IsoHardElastoplastic(mat["A"], mat["B"], mat["n"]))
     s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
     s.setDissipation(rho_b)
     g = R.Domain(mesh, m, s, track_strain=track)
     o = O.Model(mesh, mat, dict(rho_b=rho_b), mathmode=1)
@@ -194,6 +195,7 @@ def test_error_negative_jacobian_reported():
     mesh["conn"] = np.ascontiguousarray(mesh["conn"][:, [0, 3, 2, 1]])  # clockwise: detJ < 0
     m = R.Material(COPPER["E"], COPPER["nu"], COPPER["rho"], R.IsoHardElastoplastic(400e6, 100e6, 1.0))
     s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
     g = R.Domain(mesh, m, s)
     with pytest.raises(R.DnlgError, match="NEG_DETJ"):
         g.begin(1e-6)
