@@ -48,6 +48,7 @@ def main():
     ap.add_argument("--batch-engine", default="auto", choices=["auto", "persistent", "stepped", "stepped-plain"])
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-shard", action="store_true")
+    ap.add_argument("--no-pulse", action="store_true")
     ap.add_argument("--precision-report", action="store_true",
                     help="only the O-grid Johnson-Cook bar (4,000 elements per section) in both forms")
     ap.add_argument("--precision-layers", type=int, default=1000)
@@ -118,6 +119,9 @@ def main():
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg()
+    pulse = None
+    if not args.no_pulse:
+        pulse = pulse_leg(args, elapsed / args.steps * 1e3)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args)
@@ -156,6 +160,7 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "shard": shard,
+        "pulse": pulse,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -251,6 +256,42 @@ def shard_leg(args, layers=64):
     return {"workload": "hex8 O-grid 2,000 el/section x %d layers (1/8 of the 1M bar + 1 halo layer), FP64" % layers,
             "elements": ne, "element_steps_per_s": ne * args.steps / el, "ms_per_step": el / args.steps * 1e3,
             "element_kernel_ms": ms_el / max(n_el, 1), "nodal_kernel_ms": ms_nd / max(n_nd, 1)}
+
+
+def pulse_leg(args, courant_ms_per_step, steps=200):
+    """The same 1M hex8 bar with the Pulse time step (the reference's default
+    for ExplicitSolverCH, FE/ExplicitSolverCH.C:83): the cold first call of
+    the preamble, then `steps` increments with a warm call every 10."""
+    import redynela_amd as R
+    from redynela_amd import _lib as L
+    mesh = R.mesh.ogrid_c1(args.layers)
+    ne = mesh["conn"].shape[0]
+    mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
+    sol = R.ExplicitSolverCH()  # Pulse
+    sol.setDissipation(1.0)
+    d = R.Domain(mesh, mat, sol)
+    d.set_option(L.OPT_STORE_FINT, 0)
+    t0 = time.perf_counter()
+    d.begin(1.0)
+    cold_s = time.perf_counter() - t0
+    cold = d.pulse_info()
+    d.step(10)
+    calls0 = d.pulse_info()["calls"]
+    t0 = time.perf_counter()
+    d.step_async(steps)
+    d.synchronize()
+    el = time.perf_counter() - t0
+    info = d.pulse_info()
+    tm = d.time()
+    d.close()
+    calls = info["calls"] - calls0
+    ms = el / steps * 1e3
+    return {"workload": "hex8 O-grid, %d elements, FP64, Pulse time step" % ne,
+            "elements": ne, "element_steps_per_s": ne * steps / el, "ms_per_step": ms,
+            "preamble_s": cold_s, "cold_iterations": cold["iterations"],
+            "warm_calls": calls, "last_call_iterations": info["iterations"],
+            "ms_per_warm_call": (ms - courant_ms_per_step) * steps / max(calls, 1),
+            "dt_s": tm["dt"], "stiffness_bytes": ne * 24 * 24 * 8}
 
 
 def adjoint_leg():

@@ -127,6 +127,10 @@ typedef struct dnlg_handle dnlg_handle;
                                        reference's ExplicitSolverCH, FE/ExplicitSolverCH.C:83)       */
 #define DNLG_TS_COURANT 0
 #define DNLG_TS_PULSE 1
+#define DNLG_OPT_PLOT_HISTORY 6   /* keep the endStep history line of every increment (the .plot file,
+                                     FE/ExplicitSolverCH.C:459-461; turns kinetic energy on); read with
+                                     dnlg_plot_rows.  The device ring holds 8192 lines: dnlg_step_async
+                                     refuses to run further ahead of the last synchronisation */
 
 int dnlg_abi_version(void);
 
@@ -170,6 +174,42 @@ int dnlg_get_time(dnlg_handle *h, double *out);
  * measure.  A step that recomputes the Pulse time step waits on the host
  * for the power iteration, also under dnlg_step_async. */
 int dnlg_pulse_info(dnlg_handle *h, double *out4);
+
+/* endStep history lines (DNLG_OPT_PLOT_HISTORY): up to max_rows lines of
+ * 4 doubles (time, time step, computed time step, kinetic energy), oldest
+ * first, are moved into out; returns their number (the number waiting when
+ * out is NULL), or -DNLG_E_ARG.  FE/ExplicitSolverCH.C:459-461 writes the
+ * same four values with "%8.4E %8.4E %8.4E %8.4E\n" (no accessor in the reference). */
+long long dnlg_plot_rows(dnlg_handle *h, double *out, long long max_rows);
+
+/* HistoryFile nodal series (FE/HistoryFile.C:114-141; Structure::addHistoryFile,
+ * FE/Structure.C:601-613): at each endStep whose time tm reaches the file's
+ * next save time (from `start`, every `every`, up to stop + every when
+ * stop > 0) a row (tm, value of each item) is kept.  Items are (node index,
+ * field, component) with Node::getNodalValue's fields (FE/Node.C:326-342) as
+ * DNLG_HF_* codes and its components: 0 the norm of a vector, 1..3 one
+ * component; vector fields are read from the Current nodal field as the
+ * reference does at endStep (the previous increment's values), coordinates
+ * after the update.  init_temp is the material's initial temperature
+ * (Material::setInitTemperature) for the temperature fields.  Rows are read
+ * with dnlg_history_file_rows (1 + n_items doubles each); the rings hold
+ * 8192 rows between synchronisations, as the .plot ring. */
+#define DNLG_HF_DENSITY 0
+#define DNLG_HF_DENSITY_INC 1
+#define DNLG_HF_ENERGY 2
+#define DNLG_HF_ENERGY_INC 3
+#define DNLG_HF_SPEED 4
+#define DNLG_HF_SPEED_INC 5
+#define DNLG_HF_FORCE 6
+#define DNLG_HF_TEMPERATURE 7
+#define DNLG_HF_DISPLACEMENT_INC 8
+#define DNLG_HF_DISPLACEMENT 9
+#define DNLG_HF_COORDS 10
+#define DNLG_HF_MASS 11
+#define DNLG_HF_TEMPERATURE_INIT 12
+int dnlg_history_file_add(dnlg_handle *h, double start, double stop, double every, int n_items,
+                          const long long *nodes, const int *fields, const int *comps, double init_temp, int *id);
+long long dnlg_history_file_rows(dnlg_handle *h, int id, double *out, long long max_rows);
 
 /* HIP stream of the handle (hipStream_t), for callers that order work on it */
 void *dnlg_get_stream(dnlg_handle *h);

@@ -495,6 +495,7 @@ typedef struct
   double fmax0;
   double *ev, *Ke;
   long pulse_iters; /* iterations of the last call */
+  double plot_row[4]; /* the endStep history line of the last increment, FE/ExplicitSolverCH.C:459-461 */
   double pulse_conv; /* its last convergence measure */
 } dnlo;
 
@@ -1206,6 +1207,7 @@ int dnlo_max_frequency(dnlo *m, double *out)
 
 void dnlo_set_ts_method(dnlo *m, int method) { m->ts_method = method; }
 void dnlo_compute_mass(dnlo *m) { o_mass(m); }
+void dnlo_get_plot_row(const dnlo *m, double *out) { memcpy(out, m->plot_row, sizeof m->plot_row); }
 long dnlo_pulse_iterations(const dnlo *m) { return m->pulse_iters; }
 double dnlo_pulse_fmax0(const dnlo *m) { return m->fmax0; }
 
@@ -1367,6 +1369,10 @@ int dnlo_step(dnlo *m)
   /* endStep, FE/ExplicitSolverCH.C:450-468 */
   m->t += m->dt;
   if (m->ke_on) m->ke = o_kinetic_energy(m);
+  m->plot_row[0] = m->t;
+  m->plot_row[1] = m->dt;
+  m->plot_row[2] = m->computedTimeStep;
+  m->plot_row[3] = m->ke;
   m->cur = 1 - m->cur; /* transfertNodalField, FE/Node.C:219-229 */
   int run = (m->t >= 0.0) && (m->t < (m->endTime < m->upTime ? m->endTime : m->upTime));
   if (run)

@@ -17,13 +17,19 @@ ERRORS = {1: "DNLG_E_ARG", 2: "DNLG_E_POLAR_NOCONV", 3: "DNLG_E_RR_NOCONV", 4: "
 
 F_COORDS, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_FE, F_TEMP, F_DEPS = range(15)
 OPT_KINETIC_ENERGY, OPT_STORE_FINT, OPT_TRACK_DISP, OPT_TRACK_STRAIN, OPT_TIME_STEP_METHOD = 1, 2, 3, 4, 5
+OPT_PLOT_HISTORY = 6
 TS_COURANT, TS_PULSE = 0, 1
+# Node::getNodalValue field names -> DNLG_HF_* codes; True for vector fields
+HF_FIELDS = {"density": (0, False), "densityInc": (1, False), "energy": (2, False), "energyInc": (3, False),
+             "speed": (4, True), "speedInc": (5, True), "force": (6, True), "temperature": (7, False),
+             "displacementInc": (8, True), "displacement": (9, True), "coords": (10, True), "mass": (11, False),
+             "temperatureInit": (12, False)}
 
 # every symbol include/dnlg.h declares
 EXPORTS = [
     "dnlg_abi_version", "dnlg_create", "dnlg_destroy", "dnlg_set_option", "dnlg_compute_internal_forces",
     "dnlg_begin", "dnlg_step", "dnlg_solve", "dnlg_step_async", "dnlg_synchronize", "dnlg_get_field",
-    "dnlg_set_field", "dnlg_field_size", "dnlg_get_time", "dnlg_pulse_info", "dnlg_get_stream", "dnlg_enable_kernel_timing",
+    "dnlg_set_field", "dnlg_field_size", "dnlg_get_time", "dnlg_pulse_info", "dnlg_plot_rows", "dnlg_history_file_add", "dnlg_history_file_rows", "dnlg_get_stream", "dnlg_enable_kernel_timing",
     "dnlg_kernel_stats", "dnlg_last_error", "dnlg_mesh_axi_bar", "dnlg_mesh_ogrid_sizes", "dnlg_mesh_ogrid_bar",
     "dnlg_batch_create", "dnlg_batch_destroy", "dnlg_batch_solve", "dnlg_batch_get_time", "dnlg_batch_get_field",
     "dnlg_batch_last_error", "dnlg_batch_taylor_params", "dnlg_batch_set_engine",
@@ -94,6 +100,9 @@ def lib():
             "dnlg_field_size": (ll, [vp, i]),
             "dnlg_get_time": (i, [vp, vp]),
             "dnlg_pulse_info": (i, [vp, vp]),
+            "dnlg_plot_rows": (ll, [vp, vp, ll]),
+            "dnlg_history_file_add": (i, [vp, d, d, d, i, vp, vp, vp, d, C.POINTER(i)]),
+            "dnlg_history_file_rows": (ll, [vp, i, vp, ll]),
             "dnlg_get_stream": (vp, [vp]),
             "dnlg_enable_kernel_timing": (i, [vp, i]),
             "dnlg_kernel_stats": (i, [vp, i, C.POINTER(ll), C.POINTER(d)]),

@@ -23,6 +23,7 @@ UNITS = [
     ("dnlg_f32.hip", ["-ffp-contract=fast-honor-pragmas", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("dnlg_batch.hip", ["-ffp-contract=off"]),
     ("dnlg_pulse.hip", ["-ffp-contract=off"]),
+    ("dnlg_history.hip", ["-ffp-contract=off"]),
     ("dnlg_batch_f32.hip", ["-ffp-contract=fast-honor-pragmas", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("dnlg_api.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("dnlg_mesh.cpp", ["-ffp-contract=off", "-x", "hip"]),

@@ -158,12 +158,16 @@ static NodalArgs nodal_args(dnlg_handle *h)
 
 static void free_all(dnlg_handle *h)
 {
+  for (const HistFile &f : h->hf)
+    if (f.ring) (void)hipFree(f.ring);
+  if (h->hf_dev) (void)hipFree(h->hf_dev);
+  if (h->hi_dev) (void)hipFree(h->hi_dev);
   adj_free(h->adj);
   h->adj = nullptr;
   void *ptrs[] = {h->x, h->v, h->a, h->du, h->d, h->mass, h->fint, h->fe, h->S, h->evp, h->p, h->eps, h->epsplas, h->deps,
                   h->me, h->part, h->ke_part, h->bcs_v, h->conn, h->fpos, h->deg, h->bc, h->bcs, h->st, h->T,
                   h->S32, h->evp32, h->T32, h->p32, h->fe32, h->pulse_ke, h->pulse_val, h->pulse_init,
-                  h->pulse_part, h->pulse_ctl, h->pulse_y == h->fe ? nullptr : h->pulse_y};
+                  h->pulse_part, h->pulse_ctl, h->pulse_y == h->fe ? nullptr : h->pulse_y, h->plot_ring};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -376,6 +380,7 @@ extern "C" int dnlg_destroy(dnlg_handle *h)
 
 static int finalize(dnlg_handle *h);
 static int read_state(dnlg_handle *h, StepState *s);
+static constexpr long long PLOT_CAP = 8192;  // history lines the device ring holds between synchronisations
 
 extern "C" int dnlg_set_option(dnlg_handle *h, int option, int value)
 {
@@ -421,7 +426,77 @@ extern "C" int dnlg_set_option(dnlg_handle *h, int option, int value)
     h->ts_method = value;
     break;
   }
+  case DNLG_OPT_PLOT_HISTORY:
+  {
+    int rc = finalize(h);
+    if (rc) return rc;
+    StepState s;
+    rc = read_state(h, &s);
+    if (rc) return rc;
+    if (value && !h->plot_ring)
+    {
+      DNLG_HIP(h, dalloc(&h->plot_ring, PLOT_CAP * 4));
+      h->plot_drained = s.plot_n;
+    }
+    s.plot = value ? h->plot_ring : nullptr;
+    s.plot_cap = (int)PLOT_CAP;
+    DNLG_HIP(h, hipMemcpy(h->st, &s, sizeof s, hipMemcpyHostToDevice));
+    if (value) h->ke_on = true;  // the lines carry Domain::getTotalKineticEnergy
+    break;
+  }
   default: h->err = "dnlg_set_option: unknown option"; return DNLG_E_ARG;
+  }
+  return DNLG_OK;
+}
+
+// the history lines written since the last drain, ring -> host buffer
+static int drain_plot(dnlg_handle *h, long long plot_n)
+{
+  const long long n = plot_n - h->plot_drained;
+  if (n <= 0) return DNLG_OK;
+  if (n > PLOT_CAP)
+  {
+    h->err = "history lines lost: more increments than the ring holds between synchronisations";
+    return DNLG_E_STATE;
+  }
+  const size_t old = h->plot_host.size();
+  h->plot_host.resize(old + 4 * (size_t)n);
+  const long long first = h->plot_drained % PLOT_CAP;
+  const long long n1 = n < PLOT_CAP - first ? n : PLOT_CAP - first;
+  DNLG_HIP(h, hipMemcpy(&h->plot_host[old], h->plot_ring + 4 * first, sizeof(double) * 4 * n1,
+                        hipMemcpyDeviceToHost));
+  if (n > n1)
+    DNLG_HIP(h, hipMemcpy(&h->plot_host[old + 4 * n1], h->plot_ring, sizeof(double) * 4 * (n - n1),
+                          hipMemcpyDeviceToHost));
+  h->plot_drained = plot_n;
+  return DNLG_OK;
+}
+
+// HistoryFile rows written since the last drain, rings -> host buffers
+static int drain_history(dnlg_handle *h)
+{
+  const int nf = (int)h->hf.size();
+  std::vector<HistFile> dev(nf);
+  DNLG_HIP(h, hipMemcpy(dev.data(), h->hf_dev, sizeof(HistFile) * nf, hipMemcpyDeviceToHost));
+  for (int f = 0; f < nf; f++)
+  {
+    const long long n = dev[f].rows - h->hf_drained[f];
+    if (n <= 0) continue;
+    if (n > dev[f].cap)
+    {
+      h->err = "history rows lost: more increments than the ring holds between synchronisations";
+      return DNLG_E_STATE;
+    }
+    const long long w = 1 + dev[f].count, cap = dev[f].cap;
+    std::vector<double> &out = h->hf_rows[f];
+    const size_t old = out.size();
+    out.resize(old + (size_t)(n * w));
+    const long long first = h->hf_drained[f] % cap;
+    const long long n1 = n < cap - first ? n : cap - first;
+    DNLG_HIP(h, hipMemcpy(&out[old], dev[f].ring + first * w, sizeof(double) * n1 * w, hipMemcpyDeviceToHost));
+    if (n > n1)
+      DNLG_HIP(h, hipMemcpy(&out[old + n1 * w], dev[f].ring, sizeof(double) * (n - n1) * w, hipMemcpyDeviceToHost));
+    h->hf_drained[f] = dev[f].rows;
   }
   return DNLG_OK;
 }
@@ -430,7 +505,37 @@ static int read_state(dnlg_handle *h, StepState *s)
 {
   DNLG_HIP(h, hipMemcpyAsync(s, h->st, sizeof *s, hipMemcpyDeviceToHost, h->stream));
   DNLG_HIP(h, hipStreamSynchronize(h->stream));
+  if (h->plot_ring)
+  {
+    const int rc = drain_plot(h, s->plot_n);
+    if (rc) return rc;
+  }
+  if (!h->hf.empty())
+  {
+    const int rc = drain_history(h);
+    if (rc) return rc;
+  }
+  h->drain_inc = s->increment;
   return DNLG_OK;
+}
+
+static HistArgs hist_args(dnlg_handle *h)
+{
+  HistArgs a{};
+  a.files = h->hf_dev;
+  a.items = h->hi_dev;
+  a.nfiles = (int)h->hf.size();
+  a.nitems = (int)h->hitems.size();
+  a.x = h->x;
+  a.v = h->v;
+  a.a = h->a;
+  a.du = h->du;
+  a.d = h->d;
+  a.mass = h->mass;
+  a.st = h->st;
+  a.density = h->mat.density;
+  a.T0 = h->init_temp;
+  return a;
 }
 
 static int check_status(dnlg_handle *h, const StepState &s)
@@ -643,7 +748,14 @@ static int issue_steps(dnlg_handle *h, int nsteps)
     h->err = "dnlg_step: call dnlg_begin first";
     return DNLG_E_STATE;
   }
+  const bool rings = h->plot_ring || !h->hf.empty();
+  if (rings && h->inc + nsteps - h->drain_inc > PLOT_CAP)
+  {
+    h->err = "dnlg_step_async: the history rings hold 8192 increments between synchronisations";
+    return DNLG_E_STATE;
+  }
   const NodalArgs n = nodal_args(h);
+  const HistArgs ha = hist_args(h);
   const int npart = (int)courant_blocks(h->ne), nke = (int)nodal_blocks(h->nn);
   for (int s = 0; s < nsteps; s++)
   {
@@ -663,6 +775,7 @@ static int issue_steps(dnlg_handle *h, int nsteps)
       if (rec) launch_courant_f64(h->dim, h->ea, h->part, h->stream);
       launch_dt(h->st, h->part, npart, rec, h->ke_on ? h->ke_part : nullptr, nke, 0, h->stream);
     }
+    if (ha.nfiles) launch_hist_pre(ha, h->stream);
     if (h->timing)
     {
       hipEvent_t e0 = next_event(h), e1 = next_event(h), e2 = next_event(h), e3 = next_event(h);
@@ -687,6 +800,7 @@ static int issue_steps(dnlg_handle *h, int nsteps)
       launch_elem(h, true, h->has_bcs);
       launch_nodal_f64(h->dim, n, h->track_d, h->has_bcs, h->stream);
     }
+    if (ha.nfiles) launch_hist_post(ha, h->stream);
     h->inc++;
     h->pending = true;
   }
@@ -715,17 +829,114 @@ extern "C" int dnlg_synchronize(dnlg_handle *h)
 extern "C" int dnlg_step(dnlg_handle *h, int nsteps, int *running)
 {
   if (!h || nsteps < 0) return DNLG_E_ARG;
-  int rc = issue_steps(h, nsteps);
-  if (rc) return rc;
-  rc = finalize(h);
+  // with history rings, synchronise (and drain them) at least every half ring
+  const int lim = (h->plot_ring || !h->hf.empty()) ? (int)(PLOT_CAP / 2) : nsteps;
+  StepState s;
+  int rc;
+  do
+  {
+    const int k = nsteps < lim ? nsteps : lim;
+    rc = issue_steps(h, k);
+    if (rc) return rc;
+    rc = finalize(h);
+    if (rc) return rc;
+    rc = read_state(h, &s);
+    if (rc) return rc;
+    h->inc = s.increment;
+    nsteps -= k;
+  } while (nsteps > 0 && s.running && s.status == ST_OK);
+  if (running) *running = s.running;
+  if (!s.running) h->begun = false;
+  return check_status(h, s);
+}
+
+extern "C" int dnlg_history_file_add(dnlg_handle *h, double start, double stop, double every, int n_items,
+                                     const long long *nodes, const int *fields, const int *comps,
+                                     double init_temp, int *id)
+{
+  if (!h || n_items < 0 || (n_items > 0 && (!nodes || !fields || !comps)) || !(every > 0.0)) return DNLG_E_ARG;
+  for (int k = 0; k < n_items; k++)
+  {
+    if (nodes[k] < 0 || nodes[k] >= h->nn || fields[k] < 0 || fields[k] >= HF_NFIELDS || comps[k] < 0)
+    {
+      h->err = "HistoryFile::append: bad node, field or component";
+      return DNLG_E_ARG;
+    }
+    const bool vec = fields[k] == HF_SPEED || fields[k] == HF_SPEED_INC || fields[k] == HF_FORCE ||
+                     fields[k] == HF_DISPLACEMENT_INC || fields[k] == HF_DISPLACEMENT || fields[k] == HF_COORDS;
+    if (vec && comps[k] > 3)
+    {
+      h->err = "Node::getNodalValue::get: No sense for component >3 for a vectorial quantity";
+      return DNLG_E_ARG;
+    }
+    if (!vec && comps[k] > 1)
+    {
+      h->err = "Node::getNodalValue::get: No sense for component >1 for a scalar quantity";
+      return DNLG_E_ARG;
+    }
+  }
+  // bring the rows and the device state of the present files up to date
+  int rc = finalize(h);
   if (rc) return rc;
   StepState s;
   rc = read_state(h, &s);
   if (rc) return rc;
-  h->inc = s.increment;
-  if (running) *running = s.running;
-  if (!s.running) h->begun = false;
-  return check_status(h, s);
+  if (!h->hf.empty())
+    DNLG_HIP(h, hipMemcpy(h->hf.data(), h->hf_dev, sizeof(HistFile) * h->hf.size(), hipMemcpyDeviceToHost));
+  HistFile f{};
+  f.next = start;  // HistoryFile::setSaveTime, FE/HistoryFile.C:241-259
+  f.every = every;
+  f.stop = stop;
+  f.first = (int)h->hitems.size();
+  f.count = n_items;
+  f.cap = (int)PLOT_CAP;
+  DNLG_HIP(h, dalloc(&f.ring, (size_t)PLOT_CAP * (1 + n_items)));
+  for (int k = 0; k < n_items; k++) h->hitems.push_back(HistItem{(int)nodes[k], fields[k], comps[k], 0});
+  h->hf.push_back(f);
+  h->hf_drained.push_back(0);
+  h->hf_rows.emplace_back();
+  h->init_temp = init_temp;
+  if (h->hf_dev) (void)hipFree(h->hf_dev);
+  if (h->hi_dev) (void)hipFree(h->hi_dev);
+  h->hf_dev = nullptr;
+  h->hi_dev = nullptr;
+  DNLG_HIP(h, dalloc(&h->hf_dev, h->hf.size()));
+  DNLG_HIP(h, dalloc(&h->hi_dev, h->hitems.size()));
+  DNLG_HIP(h, hipMemcpy(h->hf_dev, h->hf.data(), sizeof(HistFile) * h->hf.size(), hipMemcpyHostToDevice));
+  if (!h->hitems.empty())
+    DNLG_HIP(h, hipMemcpy(h->hi_dev, h->hitems.data(), sizeof(HistItem) * h->hitems.size(), hipMemcpyHostToDevice));
+  if (id) *id = (int)h->hf.size() - 1;
+  return DNLG_OK;
+}
+
+extern "C" long long dnlg_history_file_rows(dnlg_handle *h, int id, double *out, long long max_rows)
+{
+  if (!h || id < 0 || id >= (int)h->hf.size() || max_rows < 0) return -DNLG_E_ARG;
+  std::vector<double> &r = h->hf_rows[id];
+  const long long w = 1 + h->hf[id].count;
+  const long long avail = (long long)r.size() / w;
+  if (!out) return avail;
+  const long long n = avail < max_rows ? avail : max_rows;
+  if (n > 0)
+  {
+    memcpy(out, r.data(), sizeof(double) * w * n);
+    r.erase(r.begin(), r.begin() + w * n);
+  }
+  return n;
+}
+
+extern "C" long long dnlg_plot_rows(dnlg_handle *h, double *out, long long max_rows)
+{
+  if (!h || max_rows < 0) return -DNLG_E_ARG;
+  const long long avail = (long long)(h->plot_host.size() / 4);
+  if (!out) return avail;
+  const long long n = avail < max_rows ? avail : max_rows;
+  if (n > 0)
+  {
+    memcpy(out, h->plot_host.data(), sizeof(double) * 4 * n);
+    h->plot_host.erase(h->plot_host.begin(), h->plot_host.begin() + 4 * n);
+  }
+  return n;
 }
 
 extern "C" int dnlg_pulse_info(dnlg_handle *h, double *out4)

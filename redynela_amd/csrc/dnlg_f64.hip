@@ -302,6 +302,17 @@ __global__ __launch_bounds__(256) void k_dt(StepState *st, const double *part, i
       st->pending = 0;
       go = end_step(st);
       do_ts = go;
+      if (st->plot)
+      {
+        // the history line of endStep, FE/ExplicitSolverCH.C:459-461: time,
+        // the increment's time step, the last computed one, kinetic energy
+        double *row = st->plot + 4 * (st->plot_n % st->plot_cap);
+        row[0] = st->t;
+        row[1] = st->dt;
+        row[2] = st->computedTimeStep;
+        row[3] = st->ke;
+        st->plot_n++;
+      }
     }
     if (forced) do_ts = 1;
   }

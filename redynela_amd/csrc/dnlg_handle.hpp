@@ -3,6 +3,7 @@
 #pragma once
 #include "../../include/dnlg.h"
 #include "dnlg_internal.hpp"
+#include "dnlg_history.hpp"
 
 #include <string>
 #include <vector>
@@ -53,6 +54,19 @@ struct dnlg_handle
   int ts_method = 0;
   double *pulse_ke = nullptr, *pulse_y = nullptr, *pulse_val = nullptr, *pulse_init = nullptr,
          *pulse_part = nullptr;
+  // endStep history lines (.plot): device ring, lines drained to the host at every synchronisation
+  double *plot_ring = nullptr;
+  long long plot_drained = 0;
+  std::vector<double> plot_host;  // [rows][4] not yet read by dnlg_plot_rows
+  long long drain_inc = 0;        // increment count at the last drain of the rings
+  // HistoryFile series: device tables and rings, rows drained per file
+  std::vector<HistFile> hf;       // host copy of the tables (ring pointers, first/count)
+  std::vector<HistItem> hitems;
+  HistFile *hf_dev = nullptr;
+  HistItem *hi_dev = nullptr;
+  std::vector<long long> hf_drained;
+  std::vector<std::vector<double>> hf_rows;
+  double init_temp = 0.0;
   PulseCtl *pulse_ctl = nullptr;
   long long pulse_calls = 0, pulse_iters = 0, pulse_iters_last = 0;
   std::string err;

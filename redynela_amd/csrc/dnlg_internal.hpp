@@ -30,8 +30,10 @@ struct StepState
   int pending;       // the last issued increment still needs its endStep
   int active;        // batched stepped engine: this increment runs its nodal phase
   int ts_method;     // Solver::timeStepMethod: 0 Courant, 1 Pulse
-  int pad2;
+  int plot_cap;      // rows of the .plot ring (0: no ring)
   double pulse_f;    // Domain::getMaximumFrequency of the last Pulse computation
+  double *plot;      // [plot_cap][4] ring of endStep history lines (t, dt, computed dt, KE) or null
+  long long plot_n;  // lines written so far
 };
 
 struct ElemArgs

@@ -16,8 +16,9 @@
 //   k_pulse_begin  reset the iteration counters; start vector when fmax0 == 0
 //   k_pulse_ke     Ke of 8 elements per block, one thread per row; per point
 //                  the rows share dN and the columns of C B through LDS.
-//                  Ke is stored [block][column J][8 elements x ND rows] so
-//                  that a row's entries for one J are adjacent across threads
+//                  Ke is stored [block][column pair J/2][8 elements x ND
+//                  rows][2] so that a row's 16-byte pairs are adjacent
+//                  across threads
 //   k_pulse_mv     y = Ke x for one row per thread (dgemv order: y = 0, then
 //                  y += x_J Ke(I, J) for J in order), y into the element's
 //                  ELL slot of the node
@@ -253,9 +254,9 @@ __global__ __launch_bounds__(8 * NEN * DIM) void k_pulse_ke(PulseArgs p, ElemArg
     }
   }
   if (!valid) return;
-  double *ko = p.ke + (long long)blockIdx.x * ND * NT + t;
+  double2 *ko = reinterpret_cast<double2 *>(p.ke + (long long)blockIdx.x * ND * NT) + t;
 #pragma unroll
-  for (int J = 0; J < ND; J++) ko[(long long)J * NT] = k[J];
+  for (int J = 0; J < ND; J += 2) ko[(long long)(J / 2) * NT] = make_double2(k[J], k[J + 1]);
 }
 
 template <int NEN, int DIM>
@@ -275,13 +276,17 @@ __global__ __launch_bounds__(8 * NEN * DIM) void k_pulse_mv(PulseArgs p)
   const int node = p.conn[e * NEN + I / DIM];
   loc[el][I] = p.val[(long long)node * DIM + I % DIM] * s_invf;  // ev(i) *= 1/fmax, LA/Vector.C:652-659
   __syncthreads();
-  const double *kr = p.ke + (long long)blockIdx.x * ND * NT + t;
+  const double2 *kr = reinterpret_cast<const double2 *>(p.ke + (long long)blockIdx.x * ND * NT) + t;
+  double2 kv[ND / 2];
+#pragma unroll
+  for (int j = 0; j < ND / 2; j++) kv[j] = kr[(long long)j * NT];
   double y = 0.0;
 #pragma unroll
-  for (int J = 0; J < ND; J++)
+  for (int j = 0; j < ND / 2; j++)
   {
-    const double tj = 1.0 * loc[el][J];
-    y = y + tj * kr[(long long)J * NT];
+    const double t0 = 1.0 * loc[el][2 * j], t1 = 1.0 * loc[el][2 * j + 1];
+    y = y + t0 * kv[j].x;
+    y = y + t1 * kv[j].y;
   }
   if (valid) p.y[(long long)p.fpos[e * NEN + I / DIM] * DIM + I % DIM] = y;
 }

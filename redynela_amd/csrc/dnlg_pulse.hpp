@@ -24,8 +24,8 @@ struct PulseArgs
   const unsigned char *deg;   // [nn] slots per node
   const double *mass;         // [nn] lumped mass
   const double *ev_init;      // [nn*dim] start vector ev(0) = 1, ev(i) = ev(i-1) - 2/(noddl-1)
-  double *ke;                 // [ceil(ne/8)][nd][8*nd]
-  double *y;                  // [maxdeg][nn][dim] element products, ELL slots
+  double *ke;                 // [ceil(ne/8)][nd/2][8*nd][2]: column pairs of the rows of 8 elements
+  double *y;                  // [maxdeg][nn][dim] element products Ke x, ELL slots
   double *val;                // [nn*dim] iterate before the 1/fmax scaling
   double *part;               // [ceil(nn/256)] block maxima
   PulseCtl *ctl;

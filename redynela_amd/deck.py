@@ -23,7 +23,9 @@ addMaterial, attachMaterialToElements, attachInitialBC, attachConstantBC,
 setSaveTimes, addSolver), NodeSet and ElementSet (add(i), add(i, j),
 add(i, j, step): FE/Select.C:151-212), Select, ElQua4nAx, ElHex8n3D,
 Material, IsoHardElastoplastic, IsoHardJohnsonCook, BoundaryRestrain,
-BoundarySpeed, ExplicitSolverCH.
+BoundarySpeed, ExplicitSolverCH (Courant or Pulse time step), HistoryFile
+(nodal fields; the element fields stress/strain/... are refused) and
+Structure::addHistoryFile.
 
 `load(path)` returns a Deck (mesh arrays, material, BCs, solver and save
 times); `Deck.run()` runs the Structure::solve loop over the save times on
@@ -35,6 +37,7 @@ import re
 
 import numpy as np
 
+from . import _lib
 from ._lib import DnlgError
 
 
@@ -410,6 +413,38 @@ class SolverObj(Obj):
         self.freq = _idx(v)
 
 
+class HistoryFileObj(Obj):
+    """HistoryFile (interpretor/HistoryFile.class): append(NodeSet, String[, Indice]),
+    setName(String), setSaveTime(Real) or (Real, Real, Real)."""
+    kind = "HistoryFile"
+
+    def __init__(self):
+        self.fileName = ""
+        self.save = None
+        self.items = []  # (node numbers, field, comp)
+
+    def m_append(self, deck, ns, field, comp=0):
+        from ._lib import HF_FIELDS
+        if not isinstance(ns, NodeSet):
+            raise DeckError("HistoryFile::append: not a NodeSet")
+        if field not in HF_FIELDS:
+            raise DeckError("HistoryFile::append: field %s is not supported by this reader "
+                            "(element fields need the nodal extrapolation)" % field)
+        vec = HF_FIELDS[field][1]
+        comp = _idx(comp)
+        if comp < 0 or (vec and comp > 3) or (not vec and comp > 1):
+            raise DeckError("Node::getNodalValue::get: No sense for component %d of %s" % (comp, field))
+        self.items.append((list(ns.items), field, comp))
+
+    def m_setName(self, deck, name):
+        self.fileName = name
+
+    def m_setSaveTime(self, deck, *a):
+        if len(a) not in (1, 3):
+            raise DeckError("HistoryFile::setSaveTime takes 1 or 3 arguments")
+        self.save = tuple(float(v) for v in a)
+
+
 class StructureObj(Obj):
     kind = "Structure"
 
@@ -464,9 +499,15 @@ class StructureObj(Obj):
     def m_addSolver(self, deck, s):
         deck.solver = s
 
+    def m_addHistoryFile(self, deck, hf):
+        if not isinstance(hf, HistoryFileObj):
+            raise DeckError("Structure::addHistoryFile: not a HistoryFile")
+        deck.history_files.append(hf)
+
 
 _CLASSES = {"Structure": StructureObj, "Select": Select, "NodeSet": NodeSet, "ElementSet": ElementSet,
             "Material": MaterialObj, "BoundaryRestrain": BoundaryRestrain, "BoundarySpeed": BoundarySpeed,
+            "HistoryFile": HistoryFileObj,
             "ExplicitSolverCH": SolverObj,
             "IsoHardElastoplastic": lambda: Hardening("IsoHardElastoplastic"),
             "IsoHardJohnsonCook": lambda: Hardening("IsoHardJohnsonCook"),
@@ -487,6 +528,7 @@ class Deck:
         self.initial, self.constant = [], []
         self.save = None
         self.solver = None
+        self.history_files = []
 
     # -- interpretation ----------------------------------------------------
     def execute(self, stmts):
@@ -638,6 +680,7 @@ class Deck:
                     speed[i] = v
                 else:
                     bc[i] |= (1 if v[0] == 1 else 0) | (2 if v[1] == 1 else 0) | (4 if v[2] == 1 else 0)
+        self._rank = {num: int(rank[self.node_index[num]]) for num in node_nums}
         mesh = dict(dim=self.elem_type.dim, x=x, conn=conn, bc=bc, v0=v0)
         if smask.any():
             mesh["speed_mask"], mesh["speed"] = smask, speed
@@ -657,13 +700,16 @@ class Deck:
         on_save(t, domain) is called where the reference writes a result
         file; results=<base> (or True: the deck's name) writes those files, `<base>.NNN.bin` in the
         reference's layout (redynela_amd.results), which needs the strain
-        fields and so runs with strain tracking.  Returns the Domain."""
+        fields and so runs with strain tracking; it also writes the domain's
+        `<base>_defaultDomain.plot` history (redynela_amd.results.PlotFile).
+        Returns the Domain."""
         from . import domain as D
         from . import results as RF
         mesh, mat, sol = self.model()
         if results is True:
             results = self.result_base()
         writer = RF.ResultFiles(results, mat.density, mat.init_temp) if results is not None else None
+        plot = RF.PlotFile(results + "_defaultDomain.plot") if results is not None else None
         h = mat.hardening
         hard = D.IsoHardElastoplastic(h.A, h.B, h.n) if h is not None else None
         m = D.Material(mat.young, mat.poisson, mat.density, hard)
@@ -676,10 +722,30 @@ class Deck:
         s.setPrecision(form)
         dom = D.Domain(mesh, m, s, speed_mask=mesh.get("speed_mask"), speed=mesh.get("speed"),
                        track_strain=writer is not None)
+        if plot is not None:
+            dom.set_option(_lib.OPT_PLOT_HISTORY, 1)
+        hist = []
+        for hfo in self.history_files:
+            hf = D.HistoryFile()
+            hf.setName(hfo.fileName)
+            if hfo.save is None:
+                raise DeckError("HistoryFile %s has no save time" % hfo.fileName)
+            hf.setSaveTime(*hfo.save)
+            for nums, field, comp in hfo.items:
+                hf.append([self._rank[n] for n in nums], field, comp, numbers=nums)
+            dom.addHistoryFile(hf, mat.init_temp)
+            if not hfo.fileName:  # FE/HistoryFile.C:166-171
+                raise DeckError("HistoryFile::open: No file name specified")
+            where = os.path.dirname(results) if results is not None else os.path.dirname(self.source or "")
+            hist.append((hf, RF.HistoryFileWriter(os.path.join(where, hfo.fileName), hf.columns)))
         save = self.save if self.save is not None else (0.0, sol.stop, sol.stop)
 
         def solve(up_time):
             dom.solve(up_time)
+            if plot is not None:
+                plot.append(dom.plot_rows())
+            for hf, w in hist:
+                w.append(dom.history_rows(hf))
             return dom.time()["t"]
 
         def saved(t):

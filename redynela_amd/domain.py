@@ -55,6 +55,47 @@ class IsoHardJohnsonCook(IsoHardElastoplastic):
         self.rate_and_thermal = True
 
 
+class HistoryFile:
+    """HistoryFile (FE/HistoryFile.C): nodal series written at endStep every
+    save period.  append(nodes, field, comp) takes node indices (and, for
+    the column names, their numbers), a Node::getNodalValue field name
+    (FE/Node.C:326-342) and a component (0 the norm of a vector, 1..3 one
+    component)."""
+
+    def __init__(self, name=""):
+        self.name = name
+        self.fileName = ""
+        self.start = self.every = 0.0
+        self.stop = 0.0
+        self.items = []  # (node index, field, comp, column name)
+        self.domain, self.id = None, None
+
+    def setName(self, name):
+        self.fileName = name
+
+    def setSaveTime(self, *a):
+        """(period) or (start, stop, period), FE/HistoryFile.C:241-259"""
+        if len(a) == 1:
+            self.start = self.every = float(a[0])
+            self.stop = 0.0
+        else:
+            self.start, self.stop, self.every = float(a[0]), float(a[1]), float(a[2])
+
+    def append(self, nodes, field, comp=0, numbers=None):
+        """HistoryFile::append(NodeSet*, String, Indice) (FE/HistoryFile.C:192-239)"""
+        if field not in _lib.HF_FIELDS:
+            raise DnlgError("getNodalValue: undefined field %s" % field)
+        nodes = list(nodes)
+        numbers = list(numbers) if numbers is not None else [i + 1 for i in nodes]
+        base = field + ("_%d" % comp if comp != 0 else "")
+        for i, num in zip(nodes, numbers):
+            self.items.append((int(i), field, int(comp), base + "_n%d" % num if len(nodes) > 1 else base))
+
+    @property
+    def columns(self):
+        return [it[3] for it in self.items]
+
+
 class Material:
     def __init__(self, young=0.0, poisson=0.0, density=0.0, hardening=None):
         self.young, self.poisson, self.density = float(young), float(poisson), float(density)
@@ -228,6 +269,45 @@ class Domain:
         _lib.check(_lib.lib().dnlg_get_time(self.h, out.ctypes.data), self.h)
         return dict(t=out[0], dt=out[1], increment=int(out[2]), computed_dt=out[3], ke=out[4],
                     alphaM=out[5], beta=out[6], gamma=out[7], omegaS=out[8])
+
+    def addHistoryFile(self, hf, init_temp=0.0):
+        """Structure::addHistoryFile for this domain (FE/Structure.C:601-613)."""
+        n = len(hf.items)
+        nodes = np.array([it[0] for it in hf.items], dtype=np.int64)
+        fields = np.array([_lib.HF_FIELDS[it[1]][0] for it in hf.items], dtype=np.int32)
+        comps = np.array([it[2] for it in hf.items], dtype=np.int32)
+        i = C.c_int()
+        _lib.check(_lib.lib().dnlg_history_file_add(self.h, hf.start, hf.stop, hf.every, n, nodes.ctypes.data,
+                                                    fields.ctypes.data, comps.ctypes.data, float(init_temp),
+                                                    C.byref(i)), self.h)
+        hf.domain, hf.id = self, i.value
+        return hf
+
+    def history_rows(self, hf):
+        """rows written since the last call: [n][1 + items] (time, values)."""
+        L = _lib.lib()
+        w = 1 + len(hf.items)
+        n = L.dnlg_history_file_rows(self.h, hf.id, None, 0)
+        if n < 0:
+            _lib.check(-n, self.h)
+        out = np.zeros((n, w))
+        if n:
+            got = L.dnlg_history_file_rows(self.h, hf.id, out.ctypes.data, n)
+            out = out[:got]
+        return out
+
+    def plot_rows(self):
+        """endStep history lines since the last call (DNLG_OPT_PLOT_HISTORY on):
+        [n][4] time, time step, computed time step, kinetic energy."""
+        L = _lib.lib()
+        n = L.dnlg_plot_rows(self.h, None, 0)
+        if n < 0:
+            _lib.check(-n, self.h)
+        out = np.zeros((n, 4))
+        if n:
+            got = L.dnlg_plot_rows(self.h, out.ctypes.data, n)
+            out = out[:got]
+        return out
 
     def pulse_info(self):
         """Pulse time step: calls, iterations of the last call, fmax0 and its convergence measure."""

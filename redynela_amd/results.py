@@ -162,3 +162,57 @@ class ResultFiles:
         write(path, t, nodes, ips)
         self.paths.append(path)
         return path
+
+
+PLOT_HEADER = "#DynELA_plot : Time history of computation\n#plotted : Used_timeStep timeStep kinetic_energy\n"
+
+
+class PlotFile:
+    """The domain's `.plot` history file: opened with its header by the first
+    Domain::solve (FE/Domain.C:1244-1255), then one line per increment,
+    "%8.4E %8.4E %8.4E %8.4E" of time, time step, computed time step and
+    kinetic energy (FE/ExplicitSolverCH.C:459-461).  Its name is the deck's
+    base name, '_', the domain name (defaultDomain, FE/Domain.C:55) and
+    '.plot'."""
+
+    def __init__(self, path):
+        self.path = path
+        d = os.path.dirname(path)
+        if d:
+            os.makedirs(d, exist_ok=True)
+        with open(path, "w") as f:
+            f.write(PLOT_HEADER)
+
+    def append(self, rows):
+        with open(self.path, "a") as f:
+            for r in rows:
+                f.write("%8.4E %8.4E %8.4E %8.4E\n" % (r[0], r[1], r[2], r[3]))
+
+
+def read_plot(path):
+    """[n][4] values of a .plot file (header checked)."""
+    with open(path) as f:
+        text = f.read()
+    if not text.startswith(PLOT_HEADER):
+        raise DnlgError("%s: not a DynELA .plot file" % path)
+    rows = [list(map(float, ln.split())) for ln in text[len(PLOT_HEADER):].splitlines() if ln.strip()]
+    return np.array(rows).reshape(-1, 4)
+
+
+class HistoryFileWriter:
+    """A HistoryFile's text file (FE/HistoryFile.C:114-141): header
+    "#DynELA_plot history file" and "#plotted : " with the column names, then
+    per saved increment "%10.7E " of the time and of each value."""
+
+    def __init__(self, path, columns):
+        self.path = path
+        d = os.path.dirname(path)
+        if d:
+            os.makedirs(d, exist_ok=True)
+        with open(path, "w") as f:
+            f.write("#DynELA_plot history file\n#plotted : " + "".join(c + " " for c in columns) + "\n")
+
+    def append(self, rows):
+        with open(self.path, "a") as f:
+            for r in rows:
+                f.write("".join("%10.7E " % v for v in r) + "\n")

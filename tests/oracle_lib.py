@@ -77,6 +77,7 @@ def lib():
         L.dnlo_max_frequency.argtypes = [C.c_void_p, C.c_void_p]
         L.dnlo_set_ts_method.argtypes = [C.c_void_p, C.c_int]
         L.dnlo_compute_mass.argtypes = [C.c_void_p]
+        L.dnlo_get_plot_row.argtypes = [C.c_void_p, _dp]
         L.dnlo_pulse_iterations.restype = C.c_long
         L.dnlo_pulse_iterations.argtypes = [C.c_void_p]
         L.dnlo_pulse_fmax0.restype = C.c_double
@@ -158,6 +159,12 @@ class Model:
     def solve(self, up_time):
         return lib().dnlo_solve(self.h, up_time)
 
+    def plot_row(self):
+        """(t, dt, computed dt, KE) as endStep writes them to the .plot file"""
+        out = np.zeros(4)
+        lib().dnlo_get_plot_row(self.h, out)
+        return out
+
     def hooke(self):
         C_ = np.zeros(36)
         nc = lib().dnlo_hooke(self.h, C_)
@@ -193,14 +200,14 @@ class Model:
             self.h = None
 
 
-def from_deck(d, mathmode=1):
-    """An oracle model of a redynela_amd.deck.Deck (one material, Courant step)."""
+def from_deck(d, mathmode=1, ke=False):
+    """An oracle model of a redynela_amd.deck.Deck (one material)."""
     mesh, mat, sol = d.model()
     h = mat.hardening
     m = dict(E=mat.young, nu=mat.poisson, rho=mat.density, A=h.A if h else 0.0, B=h.B if h else 0.0,
              n=h.n if h else 1.0, **({} if h else {"yield": 0}))
     s = dict(rho_b=sol.rho_b, safety=sol.safety, freq=sol.freq, end_time=sol.stop, ts_method=sol.method)
-    return Model(mesh, m, s, mathmode)
+    return Model(mesh, m, s, mathmode, ke=ke)
 
 
 def result_blocks(o, density, T0=0.0):

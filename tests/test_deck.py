@@ -168,3 +168,27 @@ def test_structure_solve_save_loop():
     # a save period of 0 asks Domain::solve for a time already reached
     with pytest.raises(deck.DeckError, match="Unable to solve problem upto time =  0.000E\\+00"):
         deck.structure_solve(1.0, (0.0, 1.0, 0.0), lambda t: t, lambda t: None)
+
+
+def test_history_file_items_and_names():
+    """HistoryFile::append column names (FE/HistoryFile.C:192-239): field,
+    _component when not 0, _n<number> when the set has several nodes"""
+    from redynela_amd import deck
+    import redynela_amd as R
+    d = deck.load(HEX)
+    d.model()
+    (hf,) = d.history_files
+    assert hf.fileName == "hex_block_history.plot" and hf.save == (1e-7, 2e-6, 2e-7)
+    h = R.HistoryFile()
+    for nums, field, comp in hf.items:
+        h.append([d._rank[n] for n in nums], field, comp, numbers=nums)
+    assert h.columns == ["speed_3_n%d" % n for n in range(28, 37)] + ["displacement", "coords_3", "density", "mass"]
+    with pytest.raises(deck.DeckError, match="stress is not supported"):
+        deck.load('Structure s; s.createNode(1,0,0,0); NodeSet n; n.add(1); HistoryFile h; h.append(n, "stress");',
+                  text=True)
+    with pytest.raises(deck.DeckError, match="No sense for component 4"):
+        deck.load('Structure s; s.createNode(1,0,0,0); NodeSet n; n.add(1); HistoryFile h; h.append(n, "speed", 4);',
+                  text=True)
+    with pytest.raises(deck.DeckError, match="No sense for component 2"):
+        deck.load('Structure s; s.createNode(1,0,0,0); NodeSet n; n.add(1); HistoryFile h; h.append(n, "mass", 2);',
+                  text=True)

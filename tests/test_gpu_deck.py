@@ -25,11 +25,13 @@ def say(*a):
 
 
 def oracle_files(d, base):
+    """the oracle through the same Structure::solve loop: result files and
+    the .plot lines (ExplicitSolverCH::solve stepped one increment at a time)"""
     from redynela_amd import deck
     from redynela_amd import results as RF
     _, mat, _ = d.model()
-    o = O.from_deck(d)
-    paths = []
+    o = O.from_deck(d, ke=True)
+    paths, rows = [], []
 
     def save(t):
         _, nodes, ips = O.result_blocks(o, mat.density, mat.init_temp)
@@ -37,11 +39,16 @@ def oracle_files(d, base):
         RF.write(paths[-1], t, nodes, ips)
 
     def solve(t):
-        assert o.solve(t) >= 0
+        r = o.begin(t)
+        assert r >= 0
+        while r == 1:
+            r = o.step()
+            assert r >= 0
+            rows.append(o.plot_row())
         return o.time()["t"]
 
     deck.structure_solve(d.solver.stop, d.save, solve, save)
-    return paths
+    return paths, np.array(rows)
 
 
 @pytest.mark.parametrize("name", ["taylor_axi_small", "hex_block"])
@@ -51,8 +58,8 @@ def test_deck_result_files_match_oracle(tmp_path, name):
     d = deck.load(os.path.join(HERE, "decks", name + ".dyn"))
     seen = []
     dom = d.run(results=str(tmp_path / "gpu" / name), on_save=lambda t, dm: seen.append(t))
-    gpu = sorted(os.listdir(tmp_path / "gpu"))
-    ref = oracle_files(d, str(tmp_path / name))
+    gpu = sorted(f for f in os.listdir(tmp_path / "gpu") if f.endswith(".bin"))
+    ref, rows = oracle_files(d, str(tmp_path / name))
     assert gpu == [os.path.basename(p) for p in ref] and len(gpu) == len(seen) >= 2
     for g, r in zip(gpu, ref):
         a = open(tmp_path / "gpu" / g, "rb").read()
@@ -63,6 +70,159 @@ def test_deck_result_files_match_oracle(tmp_path, name):
             k = np.nonzero(x.view(np.int64) != y.view(np.int64))[0]
             pytest.fail("%s: %d doubles differ, first at %d (%r vs %r)" % (g, k.size, k[0], x[k[0]], y[k[0]]))
     assert np.max(dom.evp) > 0.0
+    dom.close()
+    # the .plot history: one line per increment; time and time steps print
+    # identically, the kinetic energy (a tree sum on the GPU, a node-order sum
+    # in the oracle and the reference) to the printed 5 digits
+    from redynela_amd import results as RF
+    plot = str(tmp_path / "gpu" / (name + "_defaultDomain.plot"))
+    lines = open(plot).read().splitlines()
+    assert lines[0] == "#DynELA_plot : Time history of computation"
+    assert lines[1] == "#plotted : Used_timeStep timeStep kinetic_energy"
+    assert len(lines) - 2 == len(rows) > 10
+    for ln, r in zip(lines[2:], rows):
+        f = ln.split(" ")
+        assert f[:3] == ["%8.4E" % r[0], "%8.4E" % r[1], "%8.4E" % r[2]], ln
+        assert abs(float(f[3]) - r[3]) <= 1e-4 * abs(r[3]), ln
+    assert np.all(RF.read_plot(plot)[:, 3] > 0)
+
+
+def test_plot_rows_per_increment():
+    """the history lines against the oracle's endStep values, per increment:
+    time and time steps bit-equal, kinetic energy to 1e-13"""
+    import redynela_amd as R
+    from redynela_amd import _lib as L
+    say("case: endStep history lines, hex8 Courant, 40 increments in chunks")
+    mesh = R.mesh.ogrid_bar(6, 3, 4, 3.2e-3, 3.0e-3, 0.4, 250.0)
+    m = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
+    s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
+    g = R.Domain(mesh, m, s)
+    g.set_option(L.OPT_PLOT_HISTORY, 1)
+    o = O.Model(mesh, dict(E=115e9, nu=0.3, rho=8930.0, A=400e6, B=100e6, n=1.0), dict(), mathmode=1, ke=True)
+    g.begin(1.0)
+    o.begin(1.0)
+    want = []
+    for chunk in (1, 7, 12, 20):
+        g.step_async(chunk)
+        g.synchronize()
+        for _ in range(chunk):
+            o.step()
+            want.append(o.plot_row())
+    got = g.plot_rows()
+    want = np.array(want)
+    assert got.shape == (40, 4)
+    assert np.array_equal(got[:, :3], want[:, :3])
+    np.testing.assert_allclose(got[:, 3], want[:, 3], rtol=1e-13)
+    assert g.plot_rows().shape == (0, 4)
+    g.close()
+
+
+def oracle_history(o, items, start, stop, every, density, T0, nsteps):
+    """HistoryFile::save restated over oracle increments (FE/HistoryFile.C:114-141):
+    the Current field before the swap (read before the step), coordinates after"""
+    vec = {"speed": O.F_V, "speedInc": O.F_A, "displacementInc": O.F_DU, "displacement": O.F_D}
+    rows, nxt = [], start
+    for _ in range(nsteps):
+        cur = {k: o.get(f).reshape(-1, 3).copy() for k, f in vec.items()}
+        o.step()
+        tm, inc = o.time()["t"], o.time()["increment"]
+        if (stop > 0 and tm >= stop + every) or tm < nxt:
+            continue
+        nxt += every
+        x = o.get(O.F_X).reshape(-1, 3)
+        row = [tm]
+        for node, field, comp in items:
+            v = x[node] if field == "coords" else cur.get(field, None)
+            if field in ("coords",) or field in vec:
+                vv = v if field == "coords" else v[node]
+                row.append(np.sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]) if comp == 0 else vv[comp - 1])
+            elif field == "density":
+                row.append(density if (inc - 1) % 2 == 0 else 0.0)
+            elif field == "mass":
+                row.append(o.get(O.F_MASS)[node])
+            else:
+                raise AssertionError(field)
+        rows.append(row)
+    return np.array(rows)
+
+
+def test_history_file_rows_per_increment():
+    """HistoryFile series against the oracle, value for value"""
+    import redynela_amd as R
+    say("case: HistoryFile nodal series, hex8 Courant, 60 increments")
+    mesh = R.mesh.ogrid_bar(6, 3, 4, 3.2e-3, 3.0e-3, 0.4, 250.0)
+    m = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
+    s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
+    g = R.Domain(mesh, m, s)
+    hf = R.HistoryFile()
+    top = int(np.argmax(mesh["x"][:, 2]))
+    items = [(top, "speed", 0), (top, "speed", 3), (5, "speedInc", 1), (top, "displacement", 3),
+             (7, "displacementInc", 2), (top, "coords", 0), (top, "coords", 3), (3, "density", 0), (3, "mass", 0)]
+    for node, field, comp in items:
+        hf.append([node], field, comp)
+    hf.setSaveTime(5e-8, 1.5e-6, 1e-7)
+    g.addHistoryFile(hf)
+    o = O.Model(mesh, dict(E=115e9, nu=0.3, rho=8930.0, A=400e6, B=100e6, n=1.0), dict(), mathmode=1)
+    g.begin(1.0)
+    o.begin(1.0)
+    g.step(25)
+    g.step_async(35)
+    g.synchronize()
+    got = g.history_rows(hf)
+    want = oracle_history(o, items, 5e-8, 1.5e-6, 1e-7, 8930.0, 0.0, 60)
+    assert got.shape == want.shape and got.shape[0] >= 5, (got.shape, want.shape)
+    assert np.array_equal(got, want)
+    assert g.history_rows(hf).shape[0] == 0
+    g.close()
+
+
+def test_deck_history_file(tmp_path):
+    """the hex deck's HistoryFile, written by the deck runner, against the oracle"""
+    from redynela_amd import deck
+    say("case: deck HistoryFile text against the oracle")
+    d = deck.load(os.path.join(HERE, "decks", "hex_block.dyn"))
+    mesh, mat, sol = d.model()
+    dom = d.run(results=str(tmp_path / "hex_block"))
+    text = open(tmp_path / "hex_block_history.plot").read().splitlines()
+    assert text[0] == "#DynELA_plot history file"
+    (hfo,) = d.history_files
+    items = [(d._rank[n], f, c) for nums, f, c in hfo.items for n in nums]
+    # the oracle through the same loop
+    o = O.from_deck(d)
+    rows = []
+    start, stop, every = hfo.save
+    # ExplicitSolverCH::solve to each save time, the series taken per increment
+    nxt = start
+    vec = {"speed": O.F_V, "speedInc": O.F_A, "displacementInc": O.F_DU, "displacement": O.F_D}
+
+    def run_to(t):
+        nonlocal nxt
+        r = o.begin(t)
+        while r == 1:
+            cur = {k: o.get(f).reshape(-1, 3).copy() for k, f in vec.items()}
+            r = o.step()
+            tm, inc = o.time()["t"], o.time()["increment"]
+            if (stop > 0 and tm >= stop + every) or tm < nxt:
+                continue
+            nxt += every
+            x = o.get(O.F_X).reshape(-1, 3)
+            row = [tm]
+            for node, field, comp in items:
+                vv = x[node] if field == "coords" else (cur[field][node] if field in vec else None)
+                if vv is not None:
+                    row.append(np.sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]) if comp == 0 else vv[comp - 1])
+                elif field == "density":
+                    row.append(mat.density if (inc - 1) % 2 == 0 else 0.0)
+                else:
+                    row.append(o.get(O.F_MASS)[node])
+            rows.append(row)
+        return o.time()["t"]
+
+    deck.structure_solve(sol.stop, d.save, run_to, lambda t: None)
+    want = ["".join("%10.7E " % v for v in r) for r in rows]
+    assert text[2:] == want and len(want) >= 5
     dom.close()
 
 

@@ -121,7 +121,7 @@ def test_pulse_deck_result_files(tmp_path):
         return o.time()["t"]
 
     deck.structure_solve(d.solver.stop, d.save, solve, save)
-    gpu = sorted(os.listdir(tmp_path / "gpu"))
+    gpu = sorted(f for f in os.listdir(tmp_path / "gpu") if f.endswith(".bin"))
     assert gpu == [os.path.basename(p) for p in paths]
     for gname, p in zip(gpu, paths):
         assert open(tmp_path / "gpu" / gname, "rb").read() == open(p, "rb").read(), gname

@@ -103,3 +103,27 @@ def test_oracle_deck_run_reads_back(tmp_path):
         assert np.all(ro == (mat.density if inc % 2 == 0 else 0.0))
         assert np.all(r["nodes"][:, [4, 5, 6, 7, 14, 15, 16]] == 0.0)
     assert np.max(writer_state[-1][5]) > 0.0
+
+
+def test_plot_file_format(tmp_path):
+    """the .plot file: header of FE/Domain.C:1253-1254, lines "%8.4E %8.4E %8.4E %8.4E"
+    (FE/ExplicitSolverCH.C:459-461) as the C library prints them"""
+    import ctypes
+    from redynela_amd import results as RF
+    libc = ctypes.CDLL("libc.so.6")
+    rng = np.random.default_rng(9)
+    rows = np.concatenate([np.array([[1.5e-6, 2.25e-9, 2.5e-9, 12.345678], [3.0e-6, 1.00005e-9, 9.99995e-10, 0.0]]),
+                           10.0 ** rng.uniform(-12, 4, size=(200, 4))])
+    p = str(tmp_path / "deck_defaultDomain.plot")
+    f = RF.PlotFile(p)
+    f.append(rows)
+    want = "#DynELA_plot : Time history of computation\n#plotted : Used_timeStep timeStep kinetic_energy\n"
+    buf = ctypes.create_string_buffer(128)
+    for r in rows:
+        libc.snprintf(buf, 128, b"%8.4E %8.4E %8.4E %8.4E\n", *[ctypes.c_double(v) for v in r])
+        want += buf.value.decode()
+    text = open(p).read()
+    assert text == want
+    assert text.splitlines()[2] == "1.5000E-06 2.2500E-09 2.5000E-09 1.2346E+01"
+    r = RF.read_plot(p)
+    assert r.shape == (202, 4) and r[0, 3] == 12.346
