@@ -265,21 +265,20 @@ __global__ __launch_bounds__(8 * NEN * DIM) void k_pulse_mv(PulseArgs p)
 #pragma clang fp contract(off)
   constexpr int ND = NEN * DIM, NT = 8 * ND;
   __shared__ double loc[8][ND];
-  __shared__ double s_invf;
   if (!p.st->running || p.ctl->done) return;
   const int t = threadIdx.x, el = t / ND, I = t % ND;
   const long long e_raw = (long long)blockIdx.x * 8 + el;
   const bool valid = e_raw < p.ne;
   const long long e = valid ? e_raw : p.ne - 1;
-  if (t == 0) s_invf = p.ctl->invf;
-  __syncthreads();
-  const int node = p.conn[e * NEN + I / DIM];
-  loc[el][I] = p.val[(long long)node * DIM + I % DIM] * s_invf;  // ev(i) *= 1/fmax, LA/Vector.C:652-659
-  __syncthreads();
+  // the row of Ke first, so that its loads overlap the gather of the iterate
   const double2 *kr = reinterpret_cast<const double2 *>(p.ke + (long long)blockIdx.x * ND * NT) + t;
   double2 kv[ND / 2];
 #pragma unroll
   for (int j = 0; j < ND / 2; j++) kv[j] = kr[(long long)j * NT];
+  const double invf = p.ctl->invf;
+  const int node = p.conn[e * NEN + I / DIM];
+  loc[el][I] = p.val[(long long)node * DIM + I % DIM] * invf;  // ev(i) *= 1/fmax, LA/Vector.C:652-659
+  __syncthreads();
   double y = 0.0;
 #pragma unroll
   for (int j = 0; j < ND / 2; j++)
