@@ -9,7 +9,7 @@ hardening A = 400 MPa, B = 100 MPa, 227 m/s impact, central difference
 before the timed region; the timed region is K steps issued back to back on
 the library's stream, bracketed by device synchronisation.
 
-Prints one JSON line (see the contract in DESIGN.md): value = element-steps/s,
+Prints one JSON line: value = element-steps/s,
 roofline of the element kernel from HIP events recorded around every element
 launch of the timed region, and a CPU baseline: the oracle (C restatement of
 the reference, OpenMP element loop) on a bounded sample of the same workload.
@@ -56,7 +56,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this build runs one domain on one GPU; --gpus %d is not supported "
-                 "(the multi-GPU slab decomposition is described in DESIGN.md, not built)" % args.gpus)
+                 "(a multi-GPU slab decomposition is not built)" % args.gpus)
 
     import numpy as np
     import redynela_amd as R
@@ -70,7 +70,7 @@ def main():
         print(json.dumps({"batched": {form: batched_bars(form, engine=args.batch_engine)
                                       for form in ("fp64", "mixed")}}), flush=True)
         return
-    mesh = R.mesh.ogrid_c1(args.layers)
+    mesh = R.mesh.ogrid_taylor_bar(args.layers)
     ne, nn = mesh["conn"].shape[0], mesh["x"].shape[0]
     mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
     sol = R.ExplicitSolverCH()
@@ -234,7 +234,7 @@ def shard_leg(args, layers=64):
     FP64; the element and nodal kernels at that shape."""
     import redynela_amd as R
     from redynela_amd import _lib as L
-    mesh = R.mesh.ogrid_c1(layers)
+    mesh = R.mesh.ogrid_taylor_bar(layers)
     ne = mesh["conn"].shape[0]
     mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
     sol = R.ExplicitSolverCH()
@@ -264,7 +264,7 @@ def pulse_leg(args, courant_ms_per_step, steps=200):
     the preamble, then `steps` increments with a warm call every 10."""
     import redynela_amd as R
     from redynela_amd import _lib as L
-    mesh = R.mesh.ogrid_c1(args.layers)
+    mesh = R.mesh.ogrid_taylor_bar(args.layers)
     ne = mesh["conn"].shape[0]
     mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
     sol = R.ExplicitSolverCH()  # Pulse
@@ -300,11 +300,11 @@ def adjoint_leg():
     the reverse sweep against one forward run."""
     import numpy as np
     from redynela_amd import adjoint as AD
-    mesh = AD.c4_mesh()
+    mesh = AD.bar_20x100()
     p = AD.HardeningAdjoint(mesh)
-    xt = p.forward(AD.C4_THETA)[:, :2].copy()
+    xt = p.forward(AD.TARGET_THETA)[:, :2].copy()
     t0 = time.perf_counter()
-    x = p.forward(AD.c4_start())[:, :2]
+    x = p.forward(AD.perturbed_start())[:, :2]
     g = p.backward(2 * (x - xt))
     wall = time.perf_counter() - t0
     st = p.stats

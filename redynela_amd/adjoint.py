@@ -17,18 +17,18 @@ from .mesh import taylor_axi
 
 # the parameter-identification workload: copper, Johnson-Cook hardening
 # without rate or temperature terms, 20 x 100 quads, 200 m/s, 4,000 increments
-C4_THETA = np.array([90e6, 292e6, 0.31])
-C4_STEPS, C4_CKPT = 4000, 100
+TARGET_THETA = np.array([90e6, 292e6, 0.31])
+ADJ_STEPS, ADJ_CKPT = 4000, 100
 
 
-def c4_mesh():
+def bar_20x100():
     return taylor_axi(20, 100, 3.2e-3, 32.4e-3, 200.0)
 
 
-def c4_start(seed=20261203):
+def perturbed_start(seed=20261203):
     """(A, B, n) perturbed by a factor 1 + U(-0.2, 0.2) each (numpy PCG64 from seed)."""
     u = np.random.Generator(np.random.PCG64(seed)).uniform(-0.2, 0.2, 3)
-    return C4_THETA * (1.0 + u)
+    return TARGET_THETA * (1.0 + u)
 
 
 class HardeningAdjoint:
@@ -36,7 +36,7 @@ class HardeningAdjoint:
     the initial state, backward(xbar) returns dL/dtheta for the adjoint xbar
     of the final coordinates."""
 
-    def __init__(self, mesh, E=115e9, nu=0.3, rho=8930.0, nsteps=C4_STEPS, ckpt_every=C4_CKPT, rho_b=1.0):
+    def __init__(self, mesh, E=115e9, nu=0.3, rho=8930.0, nsteps=ADJ_STEPS, ckpt_every=ADJ_CKPT, rho_b=1.0):
         self.mesh, self.E, self.nu, self.rho = mesh, E, nu, rho
         self.nsteps, self.ckpt_every, self.rho_b = int(nsteps), int(ckpt_every), rho_b
         self.dom = None

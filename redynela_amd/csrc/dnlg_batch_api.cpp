@@ -93,7 +93,7 @@ static void bfree(dnlg_batch *b)
 
 extern "C" const char *dnlg_batch_last_error(dnlg_batch *b) { return b ? b->err.c_str() : g_berr.c_str(); }
 
-// C3 workload parameters: bar b draws, in this order, nr ~ U{5..20},
+// batched-bar workload parameters: bar b draws, in this order, nr ~ U{5..20},
 // v0 ~ U(150, 300) m/s, sigma_y0 ~ U(300, 500) MPa, hardening modulus ~ U(50, 300) MPa
 // from std::mt19937_64(seed); U(a,b) = a + (b-a) * (draw >> 11) * 2^-53.
 extern "C" int dnlg_batch_taylor_params(int n, unsigned long long seed, int *nr, double *v0, double *sy0, double *hmod)

@@ -42,15 +42,15 @@ def ogrid_bar(n, m, nz, radius=3.2e-3, length=32.4e-3, s_frac=0.4, v0=227.0):
     return dict(dim=3, x=x, conn=conn, bc=bc, v0=vel)
 
 
-# workloads quoted by the benchmark configurations
+# the Taylor-bar workloads of the tests and the benchmark
 COPPER = dict(E=115e9, nu=0.3, rho=8930.0, A=400e6, B=100e6, n=1.0)
 
 
-def taylor_c0():
+def taylor_axi_10x50():
     """Axisymmetric Taylor bar, 10 x 50 quads, 227 m/s (500 elements, 561 nodes)."""
     return taylor_axi(10, 50, 3.2e-3, 32.4e-3, 227.0)
 
 
-def ogrid_c1(nz=500):
+def ogrid_taylor_bar(nz=500):
     """3D hex8 O-grid Taylor bar: 2,000 elements per section (20x20 core + 4 rings of 20x20) x nz layers."""
     return ogrid_bar(20, 20, nz, 3.2e-3, 32.4e-3, 0.4, 227.0)

@@ -101,15 +101,15 @@ def test_adjoint_equals_numpy_restatement():
     p.close()
 
 
-def test_adjoint_c4_matches_oracle_central_differences():
+def test_adjoint_bar_20x100_matches_oracle_central_differences():
     from redynela_amd import adjoint as AD
-    mesh = AD.c4_mesh()
-    nst = AD.C4_STEPS
+    mesh = AD.bar_20x100()
+    nst = AD.ADJ_STEPS
     say("case: axi 20x100, 200 m/s, %d increments, checkpoints every %d; target (A,B,n) = (90 MPa, 292 MPa, 0.31)"
-        % (nst, AD.C4_CKPT))
-    xt = oracle_final(mesh, AD.C4_THETA, nst)
-    th = AD.c4_start()
-    p = AD.HardeningAdjoint(mesh, nsteps=nst, ckpt_every=AD.C4_CKPT)
+        % (nst, AD.ADJ_CKPT))
+    xt = oracle_final(mesh, AD.TARGET_THETA, nst)
+    th = AD.perturbed_start()
+    p = AD.HardeningAdjoint(mesh, nsteps=nst, ckpt_every=AD.ADJ_CKPT)
     t0 = time.perf_counter()
     x = p.forward(th)[:, :2]
     g = p.backward(2 * (x - xt))
@@ -132,7 +132,7 @@ def test_adjoint_c4_matches_oracle_central_differences():
                                                         for (h, fd), r in zip(row, rel)))
         assert min(rel) <= 2e-4
     # central differences of the smoothed solver at relative step 1e-6: <= 1e-5
-    xts = oracle_final(mesh, AD.C4_THETA, nst, smooth=True)
+    xts = oracle_final(mesh, AD.TARGET_THETA, nst, smooth=True)
     loss_s = lambda xx: np.sum((xx - xts) ** 2)  # noqa: E731
     for i in range(3):
         fd = central(mesh, th, i, 1e-6, nst, loss_s, smooth=True)
@@ -149,7 +149,7 @@ def test_torch_autograd_function():
     say("case: torch.autograd through FinalShape, axi 4x16, 120 increments")
     mesh = R.mesh.taylor_axi(4, 16, 3.2e-3, 5.2e-3, 200.0)
     prob = AD.HardeningAdjoint(mesh, nsteps=120, ckpt_every=50)
-    xt = torch.from_numpy(prob.forward(AD.C4_THETA)[:, :2].copy())
+    xt = torch.from_numpy(prob.forward(AD.TARGET_THETA)[:, :2].copy())
     theta = torch.tensor([100e6, 280e6, 0.3], dtype=torch.float64, requires_grad=True)
     loss = ((AD.FinalShape.apply(theta, prob) - xt) ** 2).sum()
     loss.backward()

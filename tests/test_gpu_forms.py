@@ -89,7 +89,7 @@ def test_mixed_form_tracks_fp64(case):
     from redynela_amd import _lib as L
     if case == "axi":
         say("case: mixed form vs FP64, axi 10x50 copper to 80 us")
-        mesh, mat, t_end, nr = R.mesh.taylor_c0(), CU, 80e-6, 11
+        mesh, mat, t_end, nr = R.mesh.taylor_axi_10x50(), CU, 80e-6, 11
     else:
         say("case: mixed form vs FP64, hex8 O-grid 8x4 x 40 layers JC rate+thermal to 20 us")
         mesh, mat, t_end = R.mesh.ogrid_bar(8, 4, 40, 3.2e-3, 32.4e-3, 0.4, 250.0), JCMAT, 20e-6

@@ -30,11 +30,11 @@ def gpu_domain(mesh):
     return R.Domain(mesh, m, s)
 
 
-def test_c1_full_size_bit_parity_and_invariants():
+def test_hex8_1m_full_size_bit_parity_and_invariants():
     import redynela_amd as R
     from redynela_amd import _lib as L
-    say("case: C1 1M hex8 O-grid, FP64, preamble + 3 increments against the oracle")
-    mesh = R.mesh.ogrid_c1(500)
+    say("case: 1M hex8 O-grid, FP64, preamble + 3 increments against the oracle")
+    mesh = R.mesh.ogrid_taylor_bar(500)
     assert mesh["conn"].shape[0] == 1000000 and mesh["x"].shape[0] == 1022541
     O.lib().dnlo_set_threads(16)
     o = O.Model(mesh, COPPER, dict(rho_b=1.0), mathmode=1)

@@ -4,8 +4,8 @@ The oracle (oracle/liboracle.so, det math mode: the same deterministic log/pow
 as the GPU) restates the reference's arithmetic operation for operation, so
 the bar here is bit equality: per-step nodal internal forces, coordinates,
 equivalent plastic strain, stresses and time scalars must be identical
-doubles.  (The north-star tolerances, 1e-12 per step on Fint and 1e-9 on the
-final state, are implied.)
+doubles.  (Tolerances of 1e-12 per step on Fint and 1e-9 on the final
+state are implied.)
 """
 import hashlib
 import sys
@@ -62,10 +62,10 @@ def compare_state(g, o, what):
     assert same(eg, eo), "%s: evp differs, rel %.3e" % (what, rel(eg, eo))
 
 
-def test_c0_axisymmetric_per_step_and_final():
+def test_axi_10x50_per_step_and_final():
     import redynela_amd as R
-    say("case: C0 axi 10x50 FP64, rho_b=1, per-step bit parity for 300 steps then to 80 us")
-    mesh = R.mesh.taylor_c0()
+    say("case: axi 10x50 FP64, rho_b=1, per-step bit parity for 300 steps then to 80 us")
+    mesh = R.mesh.taylor_axi_10x50()
     g, o = make_pair(mesh, COPPER)
     assert g.begin(80e-6) and o.begin(80e-6) == 1
     compare_state(g, o, "preamble")
@@ -92,10 +92,10 @@ def test_c0_axisymmetric_per_step_and_final():
     say("  final ok, increments", tg["increment"])
 
 
-def test_c0_generalized_alpha_rho_08182():
+def test_axi_10x50_generalized_alpha_rho_08182():
     import redynela_amd as R
-    say("case: C0 axi, rho_b=0.8182 (reference default), 400 steps")
-    mesh = R.mesh.taylor_c0()
+    say("case: axi 10x50, rho_b=0.8182 (reference default), 400 steps")
+    mesh = R.mesh.taylor_axi_10x50()
     g, o = make_pair(mesh, COPPER, rho_b=0.8182)
     g.begin(80e-6)
     o.begin(80e-6)

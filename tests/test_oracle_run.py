@@ -89,9 +89,9 @@ def test_ogrid_fourfold_symmetry():
     assert np.max(np.abs(x[img] - x @ rot.T)) < 1e-9 * scale
 
 
-def test_det_and_libm_math_agree_on_taylor_c0():
+def test_det_and_libm_math_agree_on_taylor_axi_10x50():
     import redynela_amd as R
-    mesh = R.mesh.taylor_c0()
+    mesh = R.mesh.taylor_axi_10x50()
     res = []
     for mode in (0, 1):
         m = O.Model(mesh, CU, dict(rho_b=1.0), mathmode=mode)
@@ -102,9 +102,9 @@ def test_det_and_libm_math_agree_on_taylor_c0():
     assert np.max(np.abs(res[0][2] - res[1][2])) <= 1e-9 * res[0][2].max()
 
 
-def test_taylor_c0_final_shape_and_mass():
+def test_taylor_axi_10x50_final_shape_and_mass():
     import redynela_amd as R
-    mesh = R.mesh.taylor_c0()
+    mesh = R.mesh.taylor_axi_10x50()
     m = O.Model(mesh, CU, dict(rho_b=1.0), mathmode=1, ke=True)
     m.begin(80e-6)
     M = m.get(O.F_MASS)
