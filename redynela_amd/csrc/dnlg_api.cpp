@@ -602,8 +602,8 @@ static int run_pulse(dnlg_handle *h)
   p.E = h->mat.young;
   p.nu = h->mat.poisson;
   launch_pulse_begin(p, h->stream);
-  launch_pulse_ke(h->dim, p, h->ea, h->stream);
-  int batch = 1;  // a call that starts from the last iterate often converges at once
+  launch_pulse_ke(h->dim, p, h->ea, h->stream);  // and the first iteration
+  int batch = 0;  // a call that starts from the last iterate often converges at once
   for (;;)
   {
     for (int b = 0; b < batch; b++) launch_pulse_iteration(h->dim, p, h->stream);
@@ -621,7 +621,7 @@ static int run_pulse(dnlg_handle *h)
       h->pulse_iters_last = c.iter;
       return c.failed ? check_status(h, s) : DNLG_OK;
     }
-    if (batch < 64) batch *= 2;
+    batch = batch == 0 ? 1 : (batch < 64 ? 2 * batch : 64);
   }
 }
 

@@ -10,15 +10,16 @@
 // scales the iterate by 1/fmax, until |fmax - fmax0| / fmax < 1e-4.  The
 // iterate and fmax0 carry over from one call to the next.
 //
-// Kernels (one call = pulse_ke once, then pulse_mv, pulse_node and
-// pulse_ctl per iteration; all of them return at once when the loop has
-// stopped or converged, so the host may issue iterations ahead):
+// Kernels (one call = pulse_ke with the first iteration's product, then
+// pulse_mv, pulse_node and pulse_ctl per iteration; all of them return at
+// once when the loop has stopped or converged, so the host may issue
+// iterations ahead):
 //   k_pulse_begin  reset the iteration counters; start vector when fmax0 == 0
-//   k_pulse_ke     Ke of 8 elements per block, one thread per row; per point
-//                  the rows share dN and the columns of C B through LDS.
-//                  Ke is stored [block][column pair J/2][8 elements x ND
-//                  rows][2] so that a row's 16-byte pairs are adjacent
-//                  across threads
+//   k_pulse_ke     Ke of 8 hex8 (16 quad) elements per 64-lane block, one lane
+//                  per (element, node) holding that node's rows; per point
+//                  the lanes share the columns of C B through LDS.  Ke is
+//                  stored [8 elements][column pair J/2][8 x ND rows][2] so
+//                  that a row's 16-byte pairs are adjacent across threads
 //   k_pulse_mv     y = Ke x for one row per thread (dgemv order: y = 0, then
 //                  y += x_J Ke(I, J) for J in order), y into the element's
 //                  ELL slot of the node
@@ -76,30 +77,36 @@ __global__ __launch_bounds__(256) void k_pulse_begin(PulseArgs p)
   }
 }
 
+// Ke of EPB elements per 64-lane block, one lane per (element, node): lane
+// i of an element owns the DIM rows of node i.  Per point, lane i computes
+// dN of its node and the DIM columns of C B that belong to it; the block
+// shares them through LDS and every lane adds its rows of B^T C B.  The
+// matrices go to HBM for the iterations, and the first iteration's product
+// Ke x is formed here from the rows in registers.
 template <int NEN, int DIM>
 struct PulseLds
 {
-  static constexpr int ND = NEN * DIM, NIP = NEN, NC = DIM == 3 ? 6 : 4;
-  double tab[NIP * NEN * 3];     // dN0 (and N for the quad) of the reference element
+  static constexpr int ND = NEN * DIM, NIP = NEN, NC = DIM == 3 ? 6 : 4, EPB = 64 / NEN;
+  double tab[NIP * NEN * 3];               // dN0 (and N for the quad) of the reference element
   double C[NC * NC];
-  double geo[8][NIP][DIM * DIM + 2];  // inv(J), W (with 2 pi R for the quad), R
-  double dN[8][NEN][DIM];
-  double CB[8][NC][ND];
+  double geo[EPB][NIP][DIM * DIM + 2];     // inv(J), W (with 2 pi R for the quad), R
+  double CB[EPB][NC][ND];
+  double loc[EPB][ND];                     // the iterate at the element's dofs
 };
 
 template <int NEN, int DIM>
-__global__ __launch_bounds__(8 * NEN * DIM) void k_pulse_ke(PulseArgs p, ElemArgs a)
+__global__ __launch_bounds__(64) void k_pulse_ke(PulseArgs p, ElemArgs a)
 {
 #pragma clang fp contract(off)
   using L = PulseLds<NEN, DIM>;
-  constexpr int ND = L::ND, NIP = L::NIP, NT = 8 * ND;
+  constexpr int ND = L::ND, NIP = L::NIP, NC = L::NC, EPB = L::EPB, NT8 = 8 * ND;
   __shared__ L s;
   if (!p.st->running) return;
-  const int t = threadIdx.x, el = t / ND, I = t % ND, i = I / DIM, comp = I % DIM;
-  const long long e_raw = (long long)blockIdx.x * 8 + el;
+  const int t = threadIdx.x, el = t / NEN, i = t % NEN;
+  const long long e_raw = (long long)blockIdx.x * EPB + el;
   const bool valid = e_raw < a.ne;
   const long long e = valid ? e_raw : a.ne - 1;
-  for (int k = t; k < NIP * NEN * 3; k += NT)
+  for (int k = t; k < NIP * NEN * 3; k += 64)
   {
     const int pt = k / (NEN * 3), n = (k / 3) % NEN, c = k % 3;
     if constexpr (DIM == 3)
@@ -116,12 +123,18 @@ __global__ __launch_bounds__(8 * NEN * DIM) void k_pulse_ke(PulseArgs p, ElemArg
     }
   }
   if (t == 0) hooke<DIM>(p.E, p.nu, s.C);
-  __syncthreads();
   const int *cn = a.conn + e * NEN;
-  // point geometry: FE/Element3D.C:75-105, FE/Element2D.C:84-115; thread I < NIP takes point I
-  if (I < NIP)
+  // the iterate (ev *= 1/fmax, LA/Vector.C:652-659) for the fused first product
   {
-    const int pt = I;
+    const double invf = p.ctl->invf;
+    const long long node = cn[i];
+#pragma unroll
+    for (int c = 0; c < DIM; c++) s.loc[el][DIM * i + c] = p.val[node * DIM + c] * invf;
+  }
+  __syncthreads();
+  // point geometry: FE/Element3D.C:75-105, FE/Element2D.C:84-115; lane i takes point i
+  {
+    const int pt = i;
     double *gq = s.geo[el][pt];
     if constexpr (DIM == 3)
     {
@@ -163,100 +176,92 @@ __global__ __launch_bounds__(8 * NEN * DIM) void k_pulse_ke(PulseArgs p, ElemArg
       gq[5] = R;
     }
   }
-  double k[ND];
+  double k[DIM][ND];
 #pragma unroll
-  for (int J = 0; J < ND; J++) k[J] = 0.0;
+  for (int c = 0; c < DIM; c++)
+#pragma unroll
+    for (int J = 0; J < ND; J++) k[c][J] = 0.0;
   const double *C = s.C;
   for (int pt = 0; pt < NIP; pt++)
   {
-    __syncthreads();  // geo written / previous point's dN and CB consumed
+    __syncthreads();  // geo written / previous point's CB consumed
     const double *gq = s.geo[el][pt];
-    if (comp == 0)
-    {
-      const double *g = s.tab + (pt * NEN + i) * 3;
-      double d[DIM];
-      dgemm_row<DIM>(g, gq, d);
-#pragma unroll
-      for (int c = 0; c < DIM; c++) s.dN[el][i][c] = d[c];
-    }
-    __syncthreads();
     const double W = gq[DIM * DIM];
-    // column I of C B
+    double d[DIM];
+    dgemm_row<DIM>(s.tab + (pt * NEN + i) * 3, gq, d);
+    // the DIM columns of C B of node i
+    if constexpr (DIM == 3)
     {
-      const double *d = s.dN[el][i];
-      if constexpr (DIM == 3)
-      {
-        const double d0 = d[0], d1 = d[1], d2 = d[2];
+      const double d0 = d[0], d1 = d[1], d2 = d[2];
 #pragma unroll
-        for (int r = 0; r < 6; r++)
-        {
-          double v;
-          if (comp == 0) v = (C[r * 6 + 0] * d0 + C[r * 6 + 3] * d1 + C[r * 6 + 5] * d2);
-          else if (comp == 1) v = (C[r * 6 + 1] * d1 + C[r * 6 + 3] * d0 + C[r * 6 + 4] * d2);
-          else v = (C[r * 6 + 2] * d2 + C[r * 6 + 4] * d1 + C[r * 6 + 5] * d0);
-          s.CB[el][r][I] = v;
-        }
+      for (int r = 0; r < 6; r++)
+      {
+        s.CB[el][r][3 * i] = (C[r * 6 + 0] * d0 + C[r * 6 + 3] * d1 + C[r * 6 + 5] * d2);
+        s.CB[el][r][3 * i + 1] = (C[r * 6 + 1] * d1 + C[r * 6 + 3] * d0 + C[r * 6 + 4] * d2);
+        s.CB[el][r][3 * i + 2] = (C[r * 6 + 2] * d2 + C[r * 6 + 4] * d1 + C[r * 6 + 5] * d0);
       }
-      else
-      {
-        const double d0 = d[0], d1 = d[1], R = gq[5], Ni = s.tab[(pt * NEN + i) * 3 + 2];
+    }
+    else
+    {
+      const double d0 = d[0], d1 = d[1], R = gq[5], Ni = s.tab[(pt * NEN + i) * 3 + 2];
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-        {
-          double v;
-          if (comp == 0)
-          {
-            v = (C[r * 4 + 0] * d0 + C[r * 4 + 2] * d1);
-            v += C[r * 4 + 3] * Ni / R;
-          }
-          else
-            v = (C[r * 4 + 1] * d1 + C[r * 4 + 2] * d0);
-          s.CB[el][r][I] = v;
-        }
+      for (int r = 0; r < 4; r++)
+      {
+        double v = (C[r * 4 + 0] * d0 + C[r * 4 + 2] * d1);
+        v += C[r * 4 + 3] * Ni / R;
+        s.CB[el][r][2 * i] = v;
+        s.CB[el][r][2 * i + 1] = (C[r * 4 + 1] * d1 + C[r * 4 + 2] * d0);
       }
     }
     __syncthreads();
-    // row I of B^T C B
+    // the DIM rows of node i of B^T C B
+    if constexpr (DIM == 3)
     {
-      const double *d = s.dN[el][i];
-      if constexpr (DIM == 3)
-      {
-        const double a0 = d[0], a1 = d[1], a2 = d[2];
-        const int r0 = comp == 0 ? 0 : (comp == 1 ? 1 : 2);
-        const int r1 = comp == 0 ? 3 : (comp == 1 ? 3 : 4);
-        const int r2 = comp == 0 ? 5 : (comp == 1 ? 4 : 5);
-        const double b0 = comp == 0 ? a0 : (comp == 1 ? a1 : a2);
-        const double b1 = comp == 0 ? a1 : (comp == 1 ? a0 : a1);
-        const double b2 = comp == 0 ? a2 : (comp == 1 ? a2 : a0);
+      const double a0 = d[0], a1 = d[1], a2 = d[2];
 #pragma unroll
-        for (int J = 0; J < ND; J++)
-          k[J] += (b0 * s.CB[el][r0][J] + b1 * s.CB[el][r1][J] + b2 * s.CB[el][r2][J]) * W;
+      for (int J = 0; J < ND; J++)
+      {
+        const double b0 = s.CB[el][0][J], b1 = s.CB[el][1][J], b2 = s.CB[el][2][J];
+        const double b3 = s.CB[el][3][J], b4 = s.CB[el][4][J], b5 = s.CB[el][5][J];
+        k[0][J] += (a0 * b0 + a1 * b3 + a2 * b5) * W;
+        k[1][J] += (a1 * b1 + a0 * b3 + a2 * b4) * W;
+        k[2][J] += (a2 * b2 + a1 * b4 + a0 * b5) * W;
       }
-      else
+    }
+    else
+    {
+      const double a0 = d[0], a1 = d[1], R = gq[5], Ni = s.tab[(pt * NEN + i) * 3 + 2];
+#pragma unroll
+      for (int J = 0; J < ND; J++)
       {
-        const double a0 = d[0], a1 = d[1];
-        if (comp == 0)
-        {
-          const double R = gq[5], Ni = s.tab[(pt * NEN + i) * 3 + 2];
-#pragma unroll
-          for (int J = 0; J < ND; J++)
-          {
-            k[J] += (a0 * s.CB[el][0][J] + a1 * s.CB[el][2][J]) * W;
-            k[J] += s.CB[el][3][J] * Ni / R * W;
-          }
-        }
-        else
-        {
-#pragma unroll
-          for (int J = 0; J < ND; J++) k[J] += (a1 * s.CB[el][1][J] + a0 * s.CB[el][2][J]) * W;
-        }
+        const double b0 = s.CB[el][0][J], b1 = s.CB[el][1][J], b2 = s.CB[el][2][J], b3 = s.CB[el][3][J];
+        k[0][J] += (a0 * b0 + a1 * b2) * W;
+        k[0][J] += b3 * Ni / R * W;
+        k[1][J] += (a1 * b1 + a0 * b2) * W;
       }
     }
   }
   if (!valid) return;
-  double2 *ko = reinterpret_cast<double2 *>(p.ke + (long long)blockIdx.x * ND * NT) + t;
+  // Ke to HBM: [e / 8][column pair][8 elements x ND rows][2]
+  double2 *ko = reinterpret_cast<double2 *>(p.ke + (e / 8) * ND * NT8);
+  const int row0 = (int)(e % 8) * ND + DIM * i;
 #pragma unroll
-  for (int J = 0; J < ND; J += 2) ko[(long long)(J / 2) * NT] = make_double2(k[J], k[J + 1]);
+  for (int c = 0; c < DIM; c++)
+#pragma unroll
+    for (int J = 0; J < ND; J += 2) ko[(long long)(J / 2) * NT8 + row0 + c] = make_double2(k[c][J], k[c][J + 1]);
+  // the first iteration's product, LAPACK dgemv order (y = 0, y += x_J Ke(I, J))
+#pragma unroll
+  for (int c = 0; c < DIM; c++)
+  {
+    double y = 0.0;
+#pragma unroll
+    for (int J = 0; J < ND; J++)
+    {
+      const double tj = 1.0 * s.loc[el][J];
+      y = y + tj * k[c][J];
+    }
+    p.y[(long long)p.fpos[e * NEN + i] * DIM + c] = y;
+  }
 }
 
 template <int NEN, int DIM>
@@ -372,9 +377,20 @@ void launch_pulse_begin(const PulseArgs &p, hipStream_t s)
 
 void launch_pulse_ke(int dim, const PulseArgs &p, const ElemArgs &a, hipStream_t s)
 {
-  const unsigned grid = (unsigned)((a.ne + 7) / 8);
-  if (dim == 3) hipLaunchKernelGGL((k_pulse_ke<8, 3>), dim3(grid), dim3(192), 0, s, p, a);
-  else hipLaunchKernelGGL((k_pulse_ke<4, 2>), dim3(grid), dim3(64), 0, s, p, a);
+  // 8 hex8 or 16 quad elements per 64-lane block; the first iteration's
+  // product is formed here, its node and control kernels follow
+  const unsigned grid_n = (unsigned)((p.nn + 255) / 256);
+  if (dim == 3)
+  {
+    hipLaunchKernelGGL((k_pulse_ke<8, 3>), dim3((unsigned)((a.ne + 7) / 8)), dim3(64), 0, s, p, a);
+    hipLaunchKernelGGL((k_pulse_node<3>), dim3(grid_n), dim3(256), 0, s, p);
+  }
+  else
+  {
+    hipLaunchKernelGGL((k_pulse_ke<4, 2>), dim3((unsigned)((a.ne + 15) / 16)), dim3(64), 0, s, p, a);
+    hipLaunchKernelGGL((k_pulse_node<2>), dim3(grid_n), dim3(256), 0, s, p);
+  }
+  hipLaunchKernelGGL(k_pulse_ctl, dim3(1), dim3(256), 0, s, p, (int)grid_n);
 }
 
 void launch_pulse_iteration(int dim, const PulseArgs &p, hipStream_t s)
