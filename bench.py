@@ -125,6 +125,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args)
+    copy = copy_peak()
     traffic = profiled("traffic_bytes_per_launch")
     valu_insts = profiled("valu_insts_per_launch")
 
@@ -148,7 +149,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_elem_hex8 (element stress/force pass)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic, "avg_ms": avg_el * 1e3, "launches": n_el,
-                     "algorithmic_bytes_per_launch": elem_bytes},
+                     "algorithmic_bytes_per_launch": elem_bytes, "copy_peak": copy},
         # the bound that applies to the bit-exact FP64 kernel: VALU issue, one
         # wave64 instruction per 4 cycles on each of 1,024 SIMDs at 2.4 GHz
         "valu_roofline": None if valu_insts is None else {
@@ -369,6 +370,34 @@ def profiled(key):
     except OSError:
         pass
     return None
+
+
+def copy_peak():
+    """Device-to-device copy of 2 GiB with torch (read + write counted), best of 10:
+    the HBM bandwidth a plain streaming kernel reaches on this box."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+        n = 1 << 28
+        a = torch.empty(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        best = 0.0
+        for _ in range(10):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            b.copy_(a)
+            e1.record()
+            e1.synchronize()
+            best = max(best, 2 * 8 * n / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+        del a, b
+        torch.cuda.empty_cache()
+        return {"GB_s": best, "bytes_per_copy": 2 * 8 * n, "kernel": "torch copy_ (float64)"}
+    except Exception:  # noqa: BLE001 - the copy probe is informative only
+        return None
 
 
 def cpu_baseline(mesh, args):
