@@ -90,8 +90,41 @@ __device__ __forceinline__ void jacobi_sym(float *a, float *V)
   a[0] = A00; a[1] = A11; a[2] = A22;
 }
 
+// symmetric (6: xx, yy, zz, xy, yz, xz) product of two commuting symmetric
+// matrices, P A with P a polynomial in A
+__device__ __forceinline__ void sym_mul(const float *P, const float *A, float *M)
+{
+  M[0] = P[0] * A[0] + P[3] * A[3] + P[5] * A[5];
+  M[1] = P[3] * A[3] + P[1] * A[1] + P[4] * A[4];
+  M[2] = P[5] * A[5] + P[4] * A[4] + P[2] * A[2];
+  M[3] = P[0] * A[3] + P[3] * A[1] + P[5] * A[4];
+  M[4] = P[3] * A[5] + P[1] * A[4] + P[4] * A[2];
+  M[5] = P[0] * A[5] + P[3] * A[4] + P[5] * A[2];
+}
+
+// f(A) = sum_k c_k A^k for symmetric A, by Horner: ((c_n A + c_{n-1} I) A + ...) A + c_0 I
+template <int N>
+__device__ __forceinline__ void sym_poly(const float *A, const float (&c)[N], float *out)
+{
+  float P[6] = {c[N - 1], c[N - 1], c[N - 1], 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = N - 2; k >= 0; k--)
+  {
+    float M[6];
+    sym_mul(P, A, M);
+    P[0] = M[0] + c[k]; P[1] = M[1] + c[k]; P[2] = M[2] + c[k];
+    P[3] = M[3]; P[4] = M[4]; P[5] = M[5];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) out[k] = P[k];
+}
+
 // Hencky increment and rotation from H = grad(du): dEps = 1/2 ln(I + E2),
 // R = F (I + E2)^(-1/2), E2 = H + H^T + H^T H.  dEps symmetric (6).
+// Small increments (every entry of E2 below 1/32 in magnitude, so that its
+// spectral radius is below 3/32) take the Taylor series of both matrix
+// functions to degree 8, whose remainder (below 3e-10) is under FP32
+// resolution; larger ones take the Jacobi eigen-decomposition.
 __device__ __forceinline__ void polar_f32(const float *H, float *dE, float *R)
 {
   float E2[6];
@@ -102,6 +135,31 @@ __device__ __forceinline__ void polar_f32(const float *H, float *dE, float *R)
   E2[3] = H[1] + H[3] + (H[0] * H[1] + H[3] * H[4] + H[6] * H[7]);
   E2[4] = H[5] + H[7] + (H[1] * H[2] + H[4] * H[5] + H[7] * H[8]);
   E2[5] = H[2] + H[6] + (H[0] * H[2] + H[3] * H[5] + H[6] * H[8]);
+  float amax = 0.f;
+#pragma unroll
+  for (int k = 0; k < 6; k++) amax = fmaxf(amax, fabsf(E2[k]));
+  if (amax < 0.03125f)
+  {
+    // 1/2 ln(1 + x) = sum_k (-1)^(k+1) x^k / (2k); (1 + x)^(-1/2) = sum_k binom(-1/2, k) x^k
+    const float cl[9] = {0.f, 0.5f, -0.25f, 1.f / 6.f, -0.125f, 0.1f, -1.f / 12.f, 1.f / 14.f, -0.0625f};
+    const float cu[9] = {1.f, -0.5f, 0.375f, -0.3125f, 0.2734375f, -0.24609375f, 0.2255859375f,
+                         -0.20947265625f, 0.196380615234375f};
+    float Ui6[6];
+    sym_poly(E2, cl, dE);
+    sym_poly(E2, cu, Ui6);
+    const float Ui[9] = {Ui6[0], Ui6[3], Ui6[5], Ui6[3], Ui6[1], Ui6[4], Ui6[5], Ui6[4], Ui6[2]};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+      {
+        float r = Ui[3 * i + j];
+#pragma unroll
+        for (int k = 0; k < 3; k++) r += H[3 * i + k] * Ui[3 * k + j];
+        R[3 * i + j] = r;
+      }
+    return;
+  }
   float V[9];
   jacobi_sym(E2, V);
   float l[3], ui[3];
