@@ -345,16 +345,18 @@ __device__ __forceinline__ void elem_hex8_f32_body(const ElemArgs32 &a, long lon
 #pragma unroll 1
   for (int pt = 0; pt < 8; pt++)
   {
+    // dN0 of the 8 nodes at this point, then (in place) dN = dN0 J^-1 for H and B^T sigma
+    float dn[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; k++) hex_dn0f(pt, k, dn[k]);
     float J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int k = 1; k < 8; k++)
     {
-      float d0[3];
-      hex_dn0f(pt, k, d0);
 #pragma unroll
       for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) J[3 * i + j] += d0[j] * xr[k][i];
+        for (int j = 0; j < 3; j++) J[3 * i + j] += dn[k][j] * xr[k][i];
     }
     const float c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8], c02 = J[3] * J[7] - J[4] * J[6];
     const float detJ = J[0] * c00 + J[1] * c01 + J[2] * c02;
@@ -368,10 +370,11 @@ __device__ __forceinline__ void elem_hex8_f32_body(const ElemArgs32 &a, long lon
 #pragma unroll
     for (int k = 0; k < 8; k++)
     {
-      float d0[3], d[3];
-      hex_dn0f(pt, k, d0);
+      float d[3];
 #pragma unroll
-      for (int j = 0; j < 3; j++) d[j] = d0[0] * inv[j] + d0[1] * inv[3 + j] + d0[2] * inv[6 + j];
+      for (int j = 0; j < 3; j++) d[j] = dn[k][0] * inv[j] + dn[k][1] * inv[3 + j] + dn[k][2] * inv[6 + j];
+#pragma unroll
+      for (int j = 0; j < 3; j++) dn[k][j] = d[j];
 #pragma unroll
       for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -398,10 +401,9 @@ __device__ __forceinline__ void elem_hex8_f32_body(const ElemArgs32 &a, long lon
 #pragma unroll
     for (int k = 0; k < 8; k++)
     {
-      float d0[3], d[3];
-      hex_dn0f(pt, k, d0);
+      float d[3];
 #pragma unroll
-      for (int j = 0; j < 3; j++) d[j] = (d0[0] * inv[j] + d0[1] * inv[3 + j] + d0[2] * inv[6 + j]) * W;
+      for (int j = 0; j < 3; j++) d[j] = dn[k][j] * W;
 #pragma unroll
       for (int i = 0; i < 3; i++)
       {
