@@ -49,6 +49,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-shard", action="store_true")
     ap.add_argument("--no-pulse", action="store_true")
+    ap.add_argument("--no-copy-peak", action="store_true")
     ap.add_argument("--precision-report", action="store_true",
                     help="only the O-grid Johnson-Cook bar (4,000 elements per section) in both forms")
     ap.add_argument("--precision-layers", type=int, default=1000)
@@ -125,7 +126,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args)
-    copy = copy_peak()
+    copy = None if args.no_copy_peak else copy_peak()
     traffic = profiled("traffic_bytes_per_launch")
     valu_insts = profiled("valu_insts_per_launch")
 
@@ -373,31 +374,74 @@ def profiled(key):
 
 
 def copy_peak():
-    """Device-to-device copy of 2 GiB with torch (read + write counted), best of 10:
-    the HBM bandwidth a plain streaming kernel reaches on this box."""
+    """Device-to-device hipMemcpy of 2 GiB (read + write counted), best of 10,
+    timed with HIP events: the HBM bandwidth a plain streaming copy reaches
+    on this box.  Called through ctypes on the HIP runtime."""
+    import ctypes as C
+    # the HIP runtime instance libdnlgpu already uses (a second copy would see no device)
+    from redynela_amd import _lib
+    _lib.lib()
+    names = []
+    with open("/proc/self/maps") as f:
+        for ln in f:
+            path = ln.split()[-1] if ln.strip() else ""
+            if "libamdhip64.so" in path and path not in names:
+                names.append(path)
+    hip = None
+    for name in names + ["libamdhip64.so"]:
+        try:
+            cand = C.CDLL(name)
+        except OSError:
+            continue
+        cnt = C.c_int(0)
+        if cand.hipGetDeviceCount(C.byref(cnt)) == 0 and cnt.value > 0:
+            hip = cand
+            break
+    if hip is None:
+        return {"error": "no HIP runtime with a device among %s" % names}
+    vp = C.c_void_p
+    hip.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
+    hip.hipMemcpyAsync.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
+    hip.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+    hip.hipMemset.argtypes = [vp, C.c_int, C.c_size_t]
+    hip.hipEventCreate.argtypes = [C.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    hip.hipFree.argtypes = [vp]
+    n = 1 << 31
+    a, b = vp(), vp()
+    e0, e1 = vp(), vp()
+    rc = hip.hipMalloc(C.byref(a), n) or hip.hipMalloc(C.byref(b), n)
+    if rc:
+        return {"error": "hipMalloc returned %d" % rc}
     try:
-        import torch
-        if not torch.cuda.is_available():
-            return None
-        n = 1 << 28
-        a = torch.empty(n, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(1.0)
-        b.copy_(a)
-        torch.cuda.synchronize()
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        hip.hipMemset(a, 1, n)
+        hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
         best = 0.0
         for _ in range(10):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            b.copy_(a)
-            e1.record()
-            e1.synchronize()
-            best = max(best, 2 * 8 * n / (e0.elapsed_time(e1) * 1e-3) / 1e9)
-        del a, b
-        torch.cuda.empty_cache()
-        return {"GB_s": best, "bytes_per_copy": 2 * 8 * n, "kernel": "torch copy_ (float64)"}
-    except Exception:  # noqa: BLE001 - the copy probe is informative only
-        return None
+            hip.hipEventRecord(e0, None)
+            hip.hipMemcpyAsync(b, a, n, 3, None)
+            hip.hipEventRecord(e1, None)
+            hip.hipEventSynchronize(e1)
+            ms = C.c_float()
+            hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+            if ms.value > 0:
+                best = max(best, 2.0 * n / (ms.value * 1e-3) / 1e9)
+        return {"GB_s": best, "bytes_per_copy": 2 * n, "how": "hipMemcpyAsync device to device, 2 GiB"}
+    finally:
+        if e0.value:
+            hip.hipEventDestroy(e0)
+        if e1.value:
+            hip.hipEventDestroy(e1)
+        if a.value:
+            hip.hipFree(a)
+        if b.value:
+            hip.hipFree(b)
 
 
 def cpu_baseline(mesh, args):
