@@ -121,10 +121,38 @@ __device__ __forceinline__ void sym_poly(const float *A, const float (&c)[N], fl
 
 // Hencky increment and rotation from H = grad(du): dEps = 1/2 ln(I + E2),
 // R = F (I + E2)^(-1/2), E2 = H + H^T + H^T H.  dEps symmetric (6).
-// Small increments (every entry of E2 below 1/32 in magnitude, so that its
-// spectral radius is below 3/32) take the Taylor series of both matrix
-// functions to degree 8, whose remainder (below 3e-10) is under FP32
-// resolution; larger ones take the Jacobi eigen-decomposition.
+// Small increments take the Taylor series of both matrix functions: to
+// degree 4 when every entry of E2 is below 2^-8 (spectral radius x below
+// 3/256: remainders x^5/10 and 0.25 x^5 below 6e-11, about x^4 relative to
+// dEps, under FP32 resolution), to degree 8 below 1/32 (x below 3/32,
+// remainders below 1.1e-10); larger increments take the Jacobi
+// eigen-decomposition.
+template <int N>
+__device__ __forceinline__ void polar_series_f32(const float *H, const float *E2, float *dE, float *R)
+{
+  // 1/2 ln(1 + x) = sum_k (-1)^(k+1) x^k / (2k); (1 + x)^(-1/2) = sum_k binom(-1/2, k) x^k
+  const float cl9[9] = {0.f, 0.5f, -0.25f, 1.f / 6.f, -0.125f, 0.1f, -1.f / 12.f, 1.f / 14.f, -0.0625f};
+  const float cu9[9] = {1.f, -0.5f, 0.375f, -0.3125f, 0.2734375f, -0.24609375f, 0.2255859375f,
+                        -0.20947265625f, 0.196380615234375f};
+  float cl[N], cu[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) { cl[k] = cl9[k]; cu[k] = cu9[k]; }
+  float Ui6[6];
+  sym_poly(E2, cl, dE);
+  sym_poly(E2, cu, Ui6);
+  const float Ui[9] = {Ui6[0], Ui6[3], Ui6[5], Ui6[3], Ui6[1], Ui6[4], Ui6[5], Ui6[4], Ui6[2]};
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+    {
+      float r = Ui[3 * i + j];
+#pragma unroll
+      for (int k = 0; k < 3; k++) r += H[3 * i + k] * Ui[3 * k + j];
+      R[3 * i + j] = r;
+    }
+}
+
 __device__ __forceinline__ void polar_f32(const float *H, float *dE, float *R)
 {
   float E2[6];
@@ -138,26 +166,14 @@ __device__ __forceinline__ void polar_f32(const float *H, float *dE, float *R)
   float amax = 0.f;
 #pragma unroll
   for (int k = 0; k < 6; k++) amax = fmaxf(amax, fabsf(E2[k]));
+  if (amax < 0.00390625f)
+  {
+    polar_series_f32<5>(H, E2, dE, R);
+    return;
+  }
   if (amax < 0.03125f)
   {
-    // 1/2 ln(1 + x) = sum_k (-1)^(k+1) x^k / (2k); (1 + x)^(-1/2) = sum_k binom(-1/2, k) x^k
-    const float cl[9] = {0.f, 0.5f, -0.25f, 1.f / 6.f, -0.125f, 0.1f, -1.f / 12.f, 1.f / 14.f, -0.0625f};
-    const float cu[9] = {1.f, -0.5f, 0.375f, -0.3125f, 0.2734375f, -0.24609375f, 0.2255859375f,
-                         -0.20947265625f, 0.196380615234375f};
-    float Ui6[6];
-    sym_poly(E2, cl, dE);
-    sym_poly(E2, cu, Ui6);
-    const float Ui[9] = {Ui6[0], Ui6[3], Ui6[5], Ui6[3], Ui6[1], Ui6[4], Ui6[5], Ui6[4], Ui6[2]};
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-      {
-        float r = Ui[3 * i + j];
-#pragma unroll
-        for (int k = 0; k < 3; k++) r += H[3 * i + k] * Ui[3 * k + j];
-        R[3 * i + j] = r;
-      }
+    polar_series_f32<9>(H, E2, dE, R);
     return;
   }
   float V[9];
