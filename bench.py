@@ -45,7 +45,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true", help="only the batched-bars leg (FP64 and mixed)")
-    ap.add_argument("--batch-engine", default="auto", choices=["auto", "persistent", "stepped", "stepped-plain"])
+    ap.add_argument("--batch-engine", default="auto", choices=["auto", "persistent", "stepped", "stepped-wide", "stepped-plain"])
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-shard", action="store_true")
     ap.add_argument("--no-pulse", action="store_true")
@@ -196,9 +196,9 @@ def mixed_form(mesh, args, fp64_value):
 
 def batched_bars(form, n=512, t_end=40e-6, engine="auto"):
     """The batched workload: 512 seeded Taylor bars (nr ~ U{5..20}, nz = 5 nr,
-    125..2,000 quads, drawn speed and hardening) advanced to 40 us by one
-    persistent launch; bars/s from the launch's HIP events, per-bar device
-    wall time (first to last instruction of the bar's workgroup)."""
+    125..2,000 quads, drawn speed and hardening) advanced to 40 us; bars/s
+    from HIP events around the whole solve, per-bar device wall time (first
+    to last instruction that advanced the bar)."""
     import numpy as np
     import redynela_amd as R
     meshes, mats = R.batch.taylor_batch(n)
@@ -206,9 +206,21 @@ def batched_bars(form, n=512, t_end=40e-6, engine="auto"):
     sol.setTimeStepMethod("Courant")
     sol.setDissipation(1.0)
     sol.setPrecision(form)
-    warm = R.Batch(meshes[:8], mats[:8], sol)  # loads the code object
-    warm.set_engine(engine)
-    warm.solve(t_end)
+    tuning = None
+    if engine == "auto":
+        # the autotuner probes the engines on this batch's first solve (48
+        # increments each, from a copy of the state); a batch of the same
+        # bars is then timed with the engine it chose, as a caller that
+        # solves batches of this shape over and over would run it
+        warm = R.Batch(meshes, mats, sol)
+        warm.solve(t_end)
+        choice, probes = warm.tuning()
+        tuning = {"chosen": choice, "probe_ms": probes, "probe_increments": 48}
+        engine = choice
+    else:
+        warm = R.Batch(meshes[:8], mats[:8], sol)  # loads the code object
+        warm.set_engine(engine)
+        warm.solve(t_end)
     warm.close()
     b = R.Batch(meshes, mats, sol)
     b.set_engine(engine)
@@ -221,7 +233,7 @@ def batched_bars(form, n=512, t_end=40e-6, engine="auto"):
     nr, v0, sy0, hm = R.batch.taylor_params(n)
     slow = [{"bar": int(i), "nr": int(nr[i]), "v0": float(v0[i]), "sigma_y0": float(sy0[i]), "H": float(hm[i]),
              "increments": int(steps[i]), "wall_us": float(wall_us[i])} for i in np.argsort(-steps)[:5]]
-    return {"bars": n, "engine": engine, "bars_per_s": n / (ms * 1e-3), "solve_ms": ms, "host_solve_s": host_s,
+    return {"bars": n, "engine": engine, "autotune": tuning, "bars_per_s": n / (ms * 1e-3), "solve_ms": ms, "host_solve_s": host_s,
             "p50_us_per_bar": float(np.percentile(wall_us, 50)), "p99_us_per_bar": float(np.percentile(wall_us, 99)),
             "max_us_per_bar": float(wall_us.max()), "element_steps": int((ne * steps).sum()),
             "element_steps_per_s": float((ne * steps).sum() / (ms * 1e-3)),

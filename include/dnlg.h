@@ -254,13 +254,22 @@ int dnlg_batch_destroy(dnlg_batch *b);
 /* engines: PERSISTENT = one workgroup carries one domain through all of its
  * increments in a single launch; STEPPED = one increment of every running
  * domain per group of launches (element pass, per-domain time step, nodal
- * phase), replayed from a captured hipGraph; AUTO (default) = STEPPED.
- * Results are bit-identical across engines. */
+ * phase) in 64-lane blocks, replayed from a captured hipGraph; STEPPED_WIDE =
+ * the same with 256-lane blocks; AUTO (default) = at the first solve, each of
+ * STEPPED, STEPPED_WIDE and PERSISTENT runs the first 48 increments from a
+ * copy of the state, the state is put back, and the fastest is kept for this
+ * and later solves of the batch.  Results are bit-identical across engines. */
 #define DNLG_BATCH_AUTO 0
 #define DNLG_BATCH_PERSISTENT 1
 #define DNLG_BATCH_STEPPED 2
 #define DNLG_BATCH_STEPPED_PLAIN 3 /* STEPPED launched increment by increment, without graphs (for tracing tools) */
+#define DNLG_BATCH_STEPPED_WIDE 4
 int dnlg_batch_set_engine(dnlg_batch *b, int engine);
+/* No counterpart in the reference (its runs are one Structure per process):
+ * the AUTO engine's choice (one of the codes above) and the device time in ms
+ * of its three probes, in the order STEPPED, STEPPED_WIDE, PERSISTENT.
+ * DNLG_E_STATE before the first solve of an AUTO batch or for another engine. */
+int dnlg_batch_get_tuning(dnlg_batch *b, int *engine, double *probe_ms3);
 /* runs every domain to up_time (or its end_time); at most max_steps increments
  * per domain (DNLG_E_STATE if a domain needs more).  wall_us[i] (optional) gets
  * domain i's device wall time from the start of its solve to the end of its

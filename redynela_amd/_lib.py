@@ -32,10 +32,10 @@ EXPORTS = [
     "dnlg_set_field", "dnlg_field_size", "dnlg_get_time", "dnlg_pulse_info", "dnlg_plot_rows", "dnlg_history_file_add", "dnlg_history_file_rows", "dnlg_get_stream", "dnlg_enable_kernel_timing",
     "dnlg_kernel_stats", "dnlg_last_error", "dnlg_mesh_axi_bar", "dnlg_mesh_ogrid_sizes", "dnlg_mesh_ogrid_bar",
     "dnlg_batch_create", "dnlg_batch_destroy", "dnlg_batch_solve", "dnlg_batch_get_time", "dnlg_batch_get_field",
-    "dnlg_batch_last_error", "dnlg_batch_taylor_params", "dnlg_batch_set_engine",
+    "dnlg_batch_last_error", "dnlg_batch_taylor_params", "dnlg_batch_set_engine", "dnlg_batch_get_tuning",
     "dnlg_adjoint_forward", "dnlg_adjoint_backward",
 ]
-BATCH_AUTO, BATCH_PERSISTENT, BATCH_STEPPED, BATCH_STEPPED_PLAIN = 0, 1, 2, 3
+BATCH_AUTO, BATCH_PERSISTENT, BATCH_STEPPED, BATCH_STEPPED_PLAIN, BATCH_STEPPED_WIDE = 0, 1, 2, 3, 4
 
 
 class Mesh(C.Structure):
@@ -118,6 +118,7 @@ def lib():
             "dnlg_batch_last_error": (C.c_char_p, [vp]),
             "dnlg_batch_taylor_params": (i, [i, C.c_ulonglong, vp, vp, vp, vp]),
             "dnlg_batch_set_engine": (i, [vp, i]),
+            "dnlg_batch_get_tuning": (i, [vp, C.POINTER(i), vp]),
             "dnlg_adjoint_forward": (i, [vp, ll, i]),
             "dnlg_adjoint_backward": (i, [vp, vp, vp, C.POINTER(AdjointStats)]),
         }

@@ -80,13 +80,25 @@ class Batch:
         if rc != 0:
             raise DnlgError("%s: %s" % (_lib.ERRORS.get(rc, rc), _lib.lib().dnlg_batch_last_error(self.h).decode()))
 
+    ENGINES = {"auto": _lib.BATCH_AUTO, "persistent": _lib.BATCH_PERSISTENT, "stepped": _lib.BATCH_STEPPED,
+               "stepped-plain": _lib.BATCH_STEPPED_PLAIN, "stepped-wide": _lib.BATCH_STEPPED_WIDE}
+
     def set_engine(self, engine):
-        """'auto', 'persistent' (one workgroup per domain, one launch),
-        'stepped' (one increment of every domain per launch group, replayed
-        from a captured graph) or 'stepped-plain' (the same without graphs)."""
-        e = {"auto": _lib.BATCH_AUTO, "persistent": _lib.BATCH_PERSISTENT, "stepped": _lib.BATCH_STEPPED,
-             "stepped-plain": _lib.BATCH_STEPPED_PLAIN}[engine]
-        self._check(_lib.lib().dnlg_batch_set_engine(self.h, e))
+        """'auto' (the default: the fastest of the next three, probed at the
+        first solve), 'persistent' (one workgroup per domain, one launch),
+        'stepped' (one increment of every domain per launch group in 64-lane
+        blocks, replayed from a captured graph), 'stepped-wide' (the same in
+        256-lane blocks) or 'stepped-plain' (stepped without graphs)."""
+        self._check(_lib.lib().dnlg_batch_set_engine(self.h, self.ENGINES[engine]))
+
+    def tuning(self):
+        """The 'auto' engine's choice after the first solve: (engine name,
+        {candidate: probe device time in ms})."""
+        e = C.c_int()
+        ms = np.zeros(3)
+        self._check(_lib.lib().dnlg_batch_get_tuning(self.h, C.byref(e), ms.ctypes.data))
+        name = {v: k for k, v in self.ENGINES.items()}[e.value]
+        return name, dict(zip(("stepped", "stepped-wide", "persistent"), ms.tolist()))
 
     def solve(self, upTime, max_steps=200000):
         """Every domain to upTime, at most max_steps increments each (a domain

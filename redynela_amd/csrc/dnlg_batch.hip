@@ -48,8 +48,8 @@ __global__ __launch_bounds__(64) void k_bstep_dt(BatchArgs B, int nb, int preamb
   if (!end_step(st)) B.t1[bar] = wall_clock64();
 }
 
-template <bool FE32>
-__global__ __launch_bounds__(64) void k_bstep_nodal(BatchArgs B, const int2 *blk)
+template <bool FE32, int NT>
+__global__ __launch_bounds__(NT) void k_bstep_nodal(BatchArgs B, const int2 *blk)
 {
   const int2 bb = blk[blockIdx.x];
   const StepState *st = B.states + bb.x;
@@ -60,10 +60,11 @@ __global__ __launch_bounds__(64) void k_bstep_nodal(BatchArgs B, const int2 *blk
   bar_node<FE32>(B.na, i, st);
 }
 
-// FP64 element pass: 16 elements (64 points) per block
-__global__ __launch_bounds__(64) void k_bstep_elem(BatchArgs B, const int2 *blk, int preamble)
+// FP64 element pass: NT / 4 elements (NT points) per block
+template <int NT>
+__global__ __launch_bounds__(NT) void k_bstep_elem(BatchArgs B, const int2 *blk, int preamble)
 {
-  __shared__ QuadLds<64> Ls;
+  __shared__ QuadLds<NT> Ls;
   const int2 bb = blk[blockIdx.x];
   StepState *st = B.states + bb.x;
   if (!st->running || st->status) return;
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(64) void k_bstep_elem(BatchArgs B, const int2 *blk,
   quad_fill_table(a, Ls);
   __syncthreads();
   const long long e_end = d.elem0 + d.ne, e_raw = (long long)bb.y + (threadIdx.x >> 2);
-  elem_quad4ax_body<false, false, false, 64>(a, e_raw, e_end, threadIdx.x, Ls);
+  elem_quad4ax_body<false, false, false, NT>(a, e_raw, e_end, threadIdx.x, Ls);
   if (preamble || (st->increment % st->freq) == 0)
   {
     double l = __builtin_inf();
@@ -87,9 +88,11 @@ void launch_batch_f64(const BatchArgs &B, int nbars, hipStream_t s)
   hipLaunchKernelGGL((k_batch_quad4<ElemPassF64, false>), dim3(nbars), dim3(256), 0, s, B);
 }
 
-void launch_bstep_elem_f64(const BatchArgs &B, const int2 *blk, int nblk, int preamble, hipStream_t s)
+// tile 0: 64-lane blocks; tile 1: 256-lane blocks (the block tables match)
+void launch_bstep_elem_f64(const BatchArgs &B, const int2 *blk, int nblk, int preamble, int tile, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_bstep_elem, dim3(nblk), dim3(64), 0, s, B, blk, preamble);
+  if (tile) hipLaunchKernelGGL(k_bstep_elem<256>, dim3(nblk), dim3(256), 0, s, B, blk, preamble);
+  else hipLaunchKernelGGL(k_bstep_elem<64>, dim3(nblk), dim3(64), 0, s, B, blk, preamble);
 }
 
 void launch_bstep_dt(const BatchArgs &B, int nb, int preamble, hipStream_t s)
@@ -97,10 +100,18 @@ void launch_bstep_dt(const BatchArgs &B, int nb, int preamble, hipStream_t s)
   hipLaunchKernelGGL(k_bstep_dt, dim3((nb + 63) / 64), dim3(64), 0, s, B, nb, preamble);
 }
 
-void launch_bstep_nodal(const BatchArgs &B, const int2 *blk, int nblk, bool fe32, hipStream_t s)
+void launch_bstep_nodal(const BatchArgs &B, const int2 *blk, int nblk, bool fe32, int tile, hipStream_t s)
 {
-  if (fe32) hipLaunchKernelGGL(k_bstep_nodal<true>, dim3(nblk), dim3(64), 0, s, B, blk);
-  else hipLaunchKernelGGL(k_bstep_nodal<false>, dim3(nblk), dim3(64), 0, s, B, blk);
+  if (tile)
+  {
+    if (fe32) hipLaunchKernelGGL((k_bstep_nodal<true, 256>), dim3(nblk), dim3(256), 0, s, B, blk);
+    else hipLaunchKernelGGL((k_bstep_nodal<false, 256>), dim3(nblk), dim3(256), 0, s, B, blk);
+  }
+  else
+  {
+    if (fe32) hipLaunchKernelGGL((k_bstep_nodal<true, 64>), dim3(nblk), dim3(64), 0, s, B, blk);
+    else hipLaunchKernelGGL((k_bstep_nodal<false, 64>), dim3(nblk), dim3(64), 0, s, B, blk);
+  }
 }
 
 }  // namespace dnl

@@ -16,10 +16,10 @@
 namespace dnl {
 void launch_batch_f64(const BatchArgs &B, int nbars, hipStream_t s);
 void launch_batch_f32(const BatchArgs &B, int nbars, hipStream_t s);
-void launch_bstep_elem_f64(const BatchArgs &B, const int2 *blk, int nblk, int preamble, hipStream_t s);
-void launch_bstep_elem_f32(const BatchArgs &B, const int2 *blk, int nblk, int preamble, hipStream_t s);
+void launch_bstep_elem_f64(const BatchArgs &B, const int2 *blk, int nblk, int preamble, int tile, hipStream_t s);
+void launch_bstep_elem_f32(const BatchArgs &B, const int2 *blk, int nblk, int preamble, int tile, hipStream_t s);
 void launch_bstep_dt(const BatchArgs &B, int nb, int preamble, hipStream_t s);
-void launch_bstep_nodal(const BatchArgs &B, const int2 *blk, int nblk, bool fe32, hipStream_t s);
+void launch_bstep_nodal(const BatchArgs &B, const int2 *blk, int nblk, bool fe32, int tile, hipStream_t s);
 void launch_mass_f64(int dim, const ElemArgs &a, double rho, const double *hn, double *me, hipStream_t s);
 void launch_gather_scalar(const unsigned char *deg, const double *src, double *dst, long long nn, hipStream_t s);
 }  // namespace dnl
@@ -49,11 +49,16 @@ struct dnlg_batch
   // compacted on the host as bars finish, and the graph is re-captured)
   int2 *blk_e = nullptr, *blk_n = nullptr;
   int nblk_e = 0, nblk_n = 0;
-  std::vector<std::vector<int2>> bar_be, bar_bn;  // every block of each bar
+  std::vector<std::vector<int2>> bar_be[2], bar_bn[2];  // every block of each bar, per tile
   unsigned long long *crit = nullptr;
   hipGraphExec_t graph = nullptr;  // kGraphSteps increments over the current tables
   int graph_nblk_e = -1, graph_nblk_n = -1;
   int engine = DNLG_BATCH_AUTO;
+  int tile = 0;                    // block tile of the stepped engine: 0 = 64 lanes, 1 = 256 lanes
+  // the autotuner's choice for DNLG_BATCH_AUTO, made at the first solve
+  bool tuned = false;
+  int auto_engine = DNLG_BATCH_STEPPED;
+  double probe_ms[3] = {0, 0, 0};  // stepped, stepped-wide, persistent
   bool mass_done = false;
   double rho = 0;
   std::string err;
@@ -297,18 +302,27 @@ extern "C" int dnlg_batch_create(int n, const dnlg_mesh *meshes, const dnlg_mate
   BCK(hipMemcpy(b->order_d, b->order.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   BCK(hipMemcpy(b->st, st.data(), sizeof(StepState) * n, hipMemcpyHostToDevice));
   {
-    // stepped engine block tables: FP64 16 elements (64 points) per block, FP32 64 elements, 64 nodes
-    const int per_e = b->form == DNLG_FORM_FP64 ? 16 : 64;
-    b->bar_be.resize(n);
-    b->bar_bn.resize(n);
+    // stepped engine block tables, per tile: blocks of 64 (tile 0) or 256 (tile 1)
+    // lanes; FP64 puts 4 points of an element on 4 lanes, FP32 one element on
+    // one lane; nodes one per lane
     long long tot_e = 0, tot_n = 0;
-    for (int i = 0; i < n; i++)
+    for (int t = 0; t < 2; t++)
     {
-      const BarDesc &d = b->bars[i];
-      for (long long e = 0; e < d.ne; e += per_e) b->bar_be[i].push_back(make_int2(i, (int)(d.elem0 + e)));
-      for (long long q = 0; q < d.nn; q += 64) b->bar_bn[i].push_back(make_int2(i, (int)(d.node0 + q)));
-      tot_e += (long long)b->bar_be[i].size();
-      tot_n += (long long)b->bar_bn[i].size();
+      const int lanes = t ? 256 : 64;
+      const int per_e = b->form == DNLG_FORM_FP64 ? lanes / 4 : lanes;
+      b->bar_be[t].resize(n);
+      b->bar_bn[t].resize(n);
+      long long te = 0, tn = 0;
+      for (int i = 0; i < n; i++)
+      {
+        const BarDesc &d = b->bars[i];
+        for (long long e = 0; e < d.ne; e += per_e) b->bar_be[t][i].push_back(make_int2(i, (int)(d.elem0 + e)));
+        for (long long q = 0; q < d.nn; q += lanes) b->bar_bn[t][i].push_back(make_int2(i, (int)(d.node0 + q)));
+        te += (long long)b->bar_be[t][i].size();
+        tn += (long long)b->bar_bn[t][i].size();
+      }
+      tot_e = std::max(tot_e, te);
+      tot_n = std::max(tot_n, tn);
     }
     std::vector<unsigned long long> inf(n, 0x7FF0000000000000ull);
     BCK(balloc(&b->blk_e, tot_e));
@@ -412,10 +426,10 @@ extern "C" int dnlg_batch_destroy(dnlg_batch *b)
 static void issue_stepped(dnlg_batch *b, int preamble)
 {
   const bool f64 = b->form == DNLG_FORM_FP64;
-  if (f64) launch_bstep_elem_f64(b->args, b->blk_e, b->nblk_e, preamble, b->stream);
-  else launch_bstep_elem_f32(b->args, b->blk_e, b->nblk_e, preamble, b->stream);
+  if (f64) launch_bstep_elem_f64(b->args, b->blk_e, b->nblk_e, preamble, b->tile, b->stream);
+  else launch_bstep_elem_f32(b->args, b->blk_e, b->nblk_e, preamble, b->tile, b->stream);
   launch_bstep_dt(b->args, b->nb, preamble, b->stream);
-  if (!preamble) launch_bstep_nodal(b->args, b->blk_n, b->nblk_n, !f64, b->stream);
+  if (!preamble) launch_bstep_nodal(b->args, b->blk_n, b->nblk_n, !f64, b->tile, b->stream);
 }
 
 // upload the block tables of the bars still running (largest bars first) and
@@ -427,8 +441,8 @@ static int compact_tables(dnlg_batch *b, const std::vector<StepState> &st, bool 
   {
     const int i = b->order[k];
     if (!st[i].running || st[i].status) continue;
-    be.insert(be.end(), b->bar_be[i].begin(), b->bar_be[i].end());
-    bn.insert(bn.end(), b->bar_bn[i].begin(), b->bar_bn[i].end());
+    be.insert(be.end(), b->bar_be[b->tile][i].begin(), b->bar_be[b->tile][i].end());
+    bn.insert(bn.end(), b->bar_bn[b->tile][i].begin(), b->bar_bn[b->tile][i].end());
   }
   if (!force && 4 * (long long)be.size() > 3 * (long long)b->graph_nblk_e) return DNLG_OK;
   b->nblk_e = (int)be.size();
@@ -490,7 +504,7 @@ static int solve_stepped(dnlg_batch *b, long long max_steps, std::vector<StepSta
 
 extern "C" int dnlg_batch_set_engine(dnlg_batch *b, int engine)
 {
-  if (!b || engine < DNLG_BATCH_AUTO || engine > DNLG_BATCH_STEPPED_PLAIN) return DNLG_E_ARG;
+  if (!b || engine < DNLG_BATCH_AUTO || engine > DNLG_BATCH_STEPPED_WIDE) return DNLG_E_ARG;
   if (b->graph)
   {
     (void)hipGraphExecDestroy(b->graph);
@@ -500,9 +514,11 @@ extern "C" int dnlg_batch_set_engine(dnlg_batch *b, int engine)
   return DNLG_OK;
 }
 
-extern "C" int dnlg_batch_solve(dnlg_batch *b, double up_time, long long max_steps, double *wall_us, double *kernel_ms)
+// one solve(up_time) of every domain with the given engine and tile
+static int run_solve(dnlg_batch *b, double up_time, long long max_steps, int engine, double *wall_us,
+                     double *kernel_ms)
 {
-  if (!b || max_steps <= 0) return DNLG_E_ARG;
+  b->tile = engine == DNLG_BATCH_STEPPED_WIDE ? 1 : 0;
   std::vector<StepState> st(b->nb);
   BHIP(hipMemcpy(st.data(), b->st, sizeof(StepState) * b->nb, hipMemcpyDeviceToHost));
   for (auto &s : st)
@@ -529,7 +545,9 @@ extern "C" int dnlg_batch_solve(dnlg_batch *b, double up_time, long long max_ste
   BHIP(hipEventCreate(&e1));
   BHIP(hipEventRecord(e0, b->stream));
   long long issued = 0;
-  const bool stepped = b->engine != DNLG_BATCH_PERSISTENT;
+  const bool stepped = engine != DNLG_BATCH_PERSISTENT;
+  const int saved_engine = b->engine;
+  b->engine = engine;  // compact_tables reads it (no graph for stepped-plain)
   if (!stepped)
   {
     if (b->form == DNLG_FORM_FP64) launch_batch_f64(b->args, b->nb, b->stream);
@@ -539,8 +557,10 @@ extern "C" int dnlg_batch_solve(dnlg_batch *b, double up_time, long long max_ste
   else
   {
     int rc = solve_stepped(b, max_steps, st, &issued);
+    b->engine = saved_engine;
     if (rc) return rc;
   }
+  b->engine = saved_engine;
   BHIP(hipEventRecord(e1, b->stream));
   BHIP(hipStreamSynchronize(b->stream));
   float ms = 0.f;
@@ -576,6 +596,121 @@ extern "C" int dnlg_batch_solve(dnlg_batch *b, double up_time, long long max_ste
       return s == ST_POLAR ? DNLG_E_POLAR_NOCONV : s == ST_RR ? DNLG_E_RR_NOCONV : s == ST_DETJ ? DNLG_E_NEG_DETJ
                                                                                             : DNLG_E_STATE;
     }
+  return DNLG_OK;
+}
+
+
+// state a solve changes, copied aside for the autotuner's probes
+struct BatchSnapshot
+{
+  std::vector<std::pair<void *, size_t>> bufs;  // (device buffer, bytes)
+  std::vector<void *> copies;
+};
+
+static int snapshot_take(dnlg_batch *b, BatchSnapshot &sn)
+{
+  const long long NN = b->NN, NIP = b->NE * 4;
+  const bool f64 = b->form == DNLG_FORM_FP64;
+  sn.bufs = {{b->x, sizeof(double) * NN * 3}, {b->v, sizeof(double) * NN * 3}, {b->a, sizeof(double) * NN * 3},
+             {b->du, sizeof(double) * NN * 3}, {b->d, sizeof(double) * NN * 3},
+             {b->st, sizeof(StepState) * b->nb}, {b->crit, sizeof(unsigned long long) * b->nb},
+             {b->t0, sizeof(unsigned long long) * b->nb}, {b->t1, sizeof(unsigned long long) * b->nb}};
+  if (f64)
+  {
+    sn.bufs.push_back({b->S, sizeof(double) * NIP * 9});
+    sn.bufs.push_back({b->evp, sizeof(double) * NIP});
+    sn.bufs.push_back({b->p, sizeof(double) * b->NE});
+  }
+  else
+  {
+    sn.bufs.push_back({b->S32, sizeof(float) * NIP * 6});
+    sn.bufs.push_back({b->evp32, sizeof(float) * NIP});
+    sn.bufs.push_back({b->p32, sizeof(float) * b->NE});
+  }
+  for (auto &q : sn.bufs)
+  {
+    void *c = nullptr;
+    BHIP(hipMalloc(&c, q.second));
+    sn.copies.push_back(c);
+    BHIP(hipMemcpyAsync(c, q.first, q.second, hipMemcpyDeviceToDevice, b->stream));
+  }
+  BHIP(hipStreamSynchronize(b->stream));
+  return DNLG_OK;
+}
+
+static int snapshot_restore(dnlg_batch *b, const BatchSnapshot &sn)
+{
+  for (size_t k = 0; k < sn.bufs.size(); k++)
+    BHIP(hipMemcpyAsync(sn.bufs[k].first, sn.copies[k], sn.bufs[k].second, hipMemcpyDeviceToDevice, b->stream));
+  BHIP(hipStreamSynchronize(b->stream));
+  return DNLG_OK;
+}
+
+static void snapshot_free(BatchSnapshot &sn)
+{
+  for (void *c : sn.copies) (void)hipFree(c);
+  sn.copies.clear();
+}
+
+// The autotuner behind DNLG_BATCH_AUTO: at the first solve, each candidate
+// (stepped engine with 64-lane blocks, stepped with 256-lane blocks,
+// persistent) runs the first kProbeSteps increments from a snapshot of the
+// state, which is restored after every probe; the candidate with the least
+// device time is kept for this and later solves.  The candidates give the
+// same results bit for bit (a block never straddles two bars and every bar's
+// arithmetic and order are fixed), so the choice changes only the time.
+static constexpr long long kProbeSteps = 48;
+
+static int autotune(dnlg_batch *b, double up_time)
+{
+  BatchSnapshot sn;
+  int rc = snapshot_take(b, sn);
+  if (rc) return rc;
+  const int cand_engine[3] = {DNLG_BATCH_STEPPED, DNLG_BATCH_STEPPED_WIDE, DNLG_BATCH_PERSISTENT};
+  int best = 0;
+  for (int c = 0; c < 3; c++)
+  {
+    double ms = 0.0;
+    rc = run_solve(b, up_time, kProbeSteps, cand_engine[c], nullptr, &ms);
+    // a probe ends on its increment budget: that status is part of the probe only
+    if (rc && rc != DNLG_E_STATE) break;
+    rc = snapshot_restore(b, sn);
+    if (rc) break;
+    b->probe_ms[c] = ms;
+    if (ms < b->probe_ms[best]) best = c;
+  }
+  const int rc2 = rc ? rc : snapshot_restore(b, sn);
+  snapshot_free(sn);
+  if (rc2) return rc2;
+  b->auto_engine = cand_engine[best];
+  b->tuned = true;
+  b->err.clear();
+  return DNLG_OK;
+}
+
+extern "C" int dnlg_batch_solve(dnlg_batch *b, double up_time, long long max_steps, double *wall_us, double *kernel_ms)
+{
+  if (!b || max_steps <= 0) return DNLG_E_ARG;
+  if (b->engine == DNLG_BATCH_AUTO && !b->tuned)
+  {
+    const int rc = autotune(b, up_time);
+    if (rc) return rc;
+  }
+  return run_solve(b, up_time, max_steps, b->engine == DNLG_BATCH_AUTO ? b->auto_engine : b->engine, wall_us,
+                   kernel_ms);
+}
+
+extern "C" int dnlg_batch_get_tuning(dnlg_batch *b, int *engine, double *probe_ms)
+{
+  if (!b) return DNLG_E_ARG;
+  if (!b->tuned)
+  {
+    b->err = "dnlg_batch_get_tuning: the autotuner has not run (engine DNLG_BATCH_AUTO, before the first solve)";
+    return DNLG_E_STATE;
+  }
+  if (engine) *engine = b->auto_engine;
+  if (probe_ms)
+    for (int c = 0; c < 3; c++) probe_ms[c] = b->probe_ms[c];
   return DNLG_OK;
 }
 

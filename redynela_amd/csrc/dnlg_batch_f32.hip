@@ -23,7 +23,8 @@ void launch_batch_f32(const BatchArgs &B, int nbars, hipStream_t s)
 }
 
 // stepped engine, mixed form: one element per lane, 64 elements per block
-__global__ __launch_bounds__(64) void k_bstep_elem32(BatchArgs B, const int2 *blk, int preamble)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_bstep_elem32(BatchArgs B, const int2 *blk, int preamble)
 {
   const int2 bb = blk[blockIdx.x];
   StepState *st = B.states + bb.x;
@@ -40,9 +41,11 @@ __global__ __launch_bounds__(64) void k_bstep_elem32(BatchArgs B, const int2 *bl
   }
 }
 
-void launch_bstep_elem_f32(const BatchArgs &B, const int2 *blk, int nblk, int preamble, hipStream_t s)
+// tile 0: 64 elements per block; tile 1: 256 (the block tables match)
+void launch_bstep_elem_f32(const BatchArgs &B, const int2 *blk, int nblk, int preamble, int tile, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_bstep_elem32, dim3(nblk), dim3(64), 0, s, B, blk, preamble);
+  if (tile) hipLaunchKernelGGL(k_bstep_elem32<256>, dim3(nblk), dim3(256), 0, s, B, blk, preamble);
+  else hipLaunchKernelGGL(k_bstep_elem32<64>, dim3(nblk), dim3(64), 0, s, B, blk, preamble);
 }
 
 }  // namespace dnl

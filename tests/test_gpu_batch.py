@@ -2,8 +2,10 @@
 
 * FP64: sampled bars of the 512-bar workload, run together to 40 us by the
   stepped engine, are bit-identical to the oracle running each bar alone, and
-  to the single-domain GPU path (also across a split solve); the persistent
-  engine gives the same bits;
+  to the single-domain GPU path (also across a split solve); every engine
+  (persistent, stepped in 64- or 256-lane blocks, without graphs, and the
+  autotuned one, whose probes must leave the state as it was) gives the same
+  bits;
 * mixed form: every bar's final length and mushroom radius within 1e-4 of
   the FP64 batch, peak plastic strain within 2e-3;
 * an exhausted increment budget is an error; a solve with nothing to do
@@ -112,20 +114,32 @@ def test_persistent_engine_equals_stepped(workload, form):
     from redynela_amd import _lib as L
     meshes, mats = workload
     idx = list(range(0, 512, 21))
-    say("case: %d bars, %s, persistent vs stepped engine (graph and plain), to 40 us" % (len(idx), form))
+    say("case: %d bars, %s, every engine (persistent, stepped 64/256 lanes, plain, auto), to 40 us"
+        % (len(idx), form))
     out = []
-    for eng in ("persistent", "stepped", "stepped-plain"):
+    for eng in ("persistent", "stepped", "stepped-wide", "stepped-plain", "auto"):
         b = R.Batch([meshes[i] for i in idx], [mats[i] for i in idx], solver(form))
         b.set_engine(eng)
+        if eng == "auto":
+            with pytest.raises(R.DnlgError, match="autotuner has not run"):
+                b.tuning()
+        # for 'auto' the probes run first, from a copy of the state that is
+        # put back, so the solve still lands on the same bits
         w, ms = b.solve(T_END)
         say("  %s: %.2f ms" % (eng, ms))
+        if eng == "auto":
+            choice, probes = b.tuning()
+            say("  auto: chose %s, probes %s" % (choice, probes))
+            assert choice in ("stepped", "stepped-wide", "persistent")
+            assert all(v > 0 for v in probes.values())
+            assert probes[choice] == min(probes.values())
         out.append([(b.time(k)["increment"], b.coords(k), b.evp(k), b.field(k, L.F_S)) for k in range(len(idx))])
         b.close()
-    for other in out[1:]:
-        for p, q in zip(out[0], other):
-            assert p[0] == q[0]
+    for name, other in zip(("stepped", "stepped-wide", "stepped-plain", "auto"), out[1:]):
+        for k, (p, q) in enumerate(zip(out[0], other)):
+            assert p[0] == q[0], (name, k)
             for u, v in zip(p[1:], q[1:]):
-                assert np.array_equal(u, v)
+                assert np.array_equal(u, v), (name, k)
 
 
 def test_batch_mixed_tracks_fp64(workload, fp64_batch):
