@@ -221,6 +221,16 @@ __device__ __forceinline__ void quad_fill_table(const ElemArgs &a, QuadLds<NT> &
   }
 }
 
+// LDS exchange among the lanes of one wave: the writes of every lane before
+// the reads of every lane (a wave's LDS operations are performed in order;
+// the fences keep the compiler from moving accesses across the barrier)
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // hex8 element kernel: lane = 8*element + integration point
 // ---------------------------------------------------------------------------
@@ -346,11 +356,13 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
       }
   }
 
-  // exchange Sig and w detJ of the 8 points of the element (dN is in place)
+  // exchange Sig and w detJ of the 8 points of the element (dN is in place);
+  // the element's 8 lanes sit in one wave, whose LDS operations complete in
+  // order, so a wave-level barrier orders the writes before the reads
 #pragma unroll
   for (int c = 0; c < 9; c++) Ls.x[9 + c][tid] = Sig.v[c];
   Ls.x[18][tid] = WxdJ;
-  __syncthreads();
+  wave_lds_sync();
 
   // nodal force of local node L = pt, accumulated point by point as
   // Element::computeInternalForces does, FE/Element.C:178-202

@@ -21,17 +21,20 @@
 
 namespace dnl {
 
-// 3 waves per SIMD for the production instances (168 VGPRs, a few spilled);
-// the strain-tracking instances are not held to it
+// 256 lanes (32 elements) per workgroup: the four waves share one table of
+// shape-function derivatives, so the LDS of a workgroup (40 KB) lets four
+// workgroups, 4 waves per SIMD, sit on a CU -- the limit the registers set
+// too; with one wave per workgroup, LDS held it to 3.25
+constexpr int kHexNT = 256;
 template <bool SPEED_BC, bool TRACK, bool JC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TRACK ? 1 : 3))) void k_elem_hex8(ElemArgs a)
+__global__ __launch_bounds__(kHexNT) void k_elem_hex8(ElemArgs a)
 {
   if (a.check_running && !a.st->running) return;
-  __shared__ HexLds<64> Ls;
+  __shared__ HexLds<kHexNT> Ls;
   hex_fill_table(a, Ls);
   __syncthreads();
-  const long long e_raw = ((long long)blockIdx.x * 64 + threadIdx.x) >> 3;
-  elem_hex8_body<SPEED_BC, TRACK, JC, 64>(a, e_raw, a.ne, threadIdx.x, Ls);
+  const long long e_raw = ((long long)blockIdx.x * kHexNT + threadIdx.x) >> 3;
+  elem_hex8_body<SPEED_BC, TRACK, JC, kHexNT>(a, e_raw, a.ne, threadIdx.x, Ls);
 }
 
 template <bool SPEED_BC, bool TRACK, bool JC>
@@ -345,12 +348,13 @@ void launch_elem_f64(int dim, const ElemArgs &a, bool speed_bc, hipStream_t s)
   if (a.ne <= 0) return;
   const bool tr = a.eps != nullptr;
   const long long lanes = a.ne * (dim == 3 ? 8 : 4);
-  const unsigned grid = (unsigned)((lanes + 63) / 64);
+  const int nt = dim == 3 ? kHexNT : 64;
+  const unsigned grid = (unsigned)((lanes + nt - 1) / nt);
 #define DNL_ELEM(K, B, T)                                                                      \
   do                                                                                           \
   {                                                                                            \
-    if (a.law == 2) hipLaunchKernelGGL((K<B, T, true>), dim3(grid), dim3(64), 0, s, a);       \
-    else hipLaunchKernelGGL((K<B, T, false>), dim3(grid), dim3(64), 0, s, a);                 \
+    if (a.law == 2) hipLaunchKernelGGL((K<B, T, true>), dim3(grid), dim3(nt), 0, s, a);       \
+    else hipLaunchKernelGGL((K<B, T, false>), dim3(grid), dim3(nt), 0, s, a);                 \
   } while (0)
   if (dim == 3)
   {
