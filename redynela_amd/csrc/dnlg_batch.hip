@@ -60,9 +60,11 @@ __global__ __launch_bounds__(NT) void k_bstep_nodal(BatchArgs B, const int2 *blk
   bar_node<FE32>(B.na, i, st);
 }
 
-// FP64 element pass: NT / 4 elements (NT points) per block
+// FP64 element pass: NT / 4 elements (NT points) per block; 5 waves per SIMD
+// (96 VGPRs, no spills; the LDS of either tile allows as many)
 template <int NT>
-__global__ __launch_bounds__(NT) void k_bstep_elem(BatchArgs B, const int2 *blk, int preamble)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_bstep_elem(BatchArgs B, const int2 *blk,
+                                                                                          int preamble)
 {
   __shared__ QuadLds<NT> Ls;
   const int2 bb = blk[blockIdx.x];

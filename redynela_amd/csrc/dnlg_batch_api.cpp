@@ -482,7 +482,21 @@ static int solve_stepped(dnlg_batch *b, long long max_steps, std::vector<StepSta
   long long done = 0, chunk = 32;
   while (done < max_steps)
   {
-    const long long n = std::min(chunk, max_steps - done);
+    long long n = std::min(chunk, max_steps - done);
+    if (done > 0)
+    {
+      // no more increments than the running bars need at their present time
+      // step (whole graph replays); a bar whose step shrinks gets another chunk
+      long long need = 0;
+      for (const StepState &s : st)
+        if (s.running && !s.status && s.dt > 0.0)
+        {
+          const double lim = s.endTime < s.upTime ? s.endTime : s.upTime;
+          need = std::max(need, (long long)std::ceil((lim - s.t) / s.dt) + 1);
+        }
+      need = (need + kGraphSteps - 1) / kGraphSteps * kGraphSteps;
+      n = std::min(n, std::max<long long>(need, kGraphSteps));
+    }
     long long k = 0;
     if (b->graph)
       for (; k + kGraphSteps <= n; k += kGraphSteps) BHIP(hipGraphLaunch(b->graph, b->stream));
