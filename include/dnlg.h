@@ -271,7 +271,11 @@ int dnlg_batch_set_engine(dnlg_batch *b, int engine);
  * DNLG_E_STATE before the first solve of an AUTO batch or for another engine. */
 int dnlg_batch_get_tuning(dnlg_batch *b, int *engine, double *probe_ms3);
 /* runs every domain to up_time (or its end_time); at most max_steps increments
- * per domain (DNLG_E_STATE if a domain needs more).  wall_us[i] (optional) gets
+ * per domain (DNLG_E_STATE if a domain needs more).  A domain stops at its
+ * first element error with the failing increment counted and its nodal state
+ * and time as before that increment (the reference stops inside
+ * computeInternalForces); the other domains run on, and the error of the
+ * lowest-numbered failed domain is returned.  wall_us[i] (optional) gets
  * domain i's device wall time from the start of its solve to the end of its
  * last increment, kernel_ms (optional) the duration of the whole solve on the
  * device stream. */

@@ -18,7 +18,8 @@ namespace dnl {
 // element pass.  Otherwise, for each running bar: computeTimeStep (Courant
 // minimum from this increment's element pass when increment % freq == 0),
 // initStep, and -- ahead of the nodal phase, which reads neither t nor the
-// running flag -- the endStep time update and loop condition.
+// running flag -- the endStep time update and loop condition.  A bar stops
+// at the first element error, with the failing increment counted.
 __global__ __launch_bounds__(64) void k_bstep_dt(BatchArgs B, int nb, int preamble)
 {
   const int bar = blockIdx.x * 64 + threadIdx.x;
@@ -29,14 +30,16 @@ __global__ __launch_bounds__(64) void k_bstep_dt(BatchArgs B, int nb, int preamb
   if (preamble)
   {
     st->active = 0;
-    if (st->running)
+    if (st->running && !st->status)
     {
       B.t0[bar] = wall_clock64();
       compute_time_step(st, 1, crit);
     }
     return;
   }
-  if (!st->running || st->status)
+  // a bar whose status was raised before this increment's element pass stays
+  // where it stopped
+  if (!st->running || (st->status && !st->active))
   {
     st->active = 0;
     return;
@@ -44,6 +47,15 @@ __global__ __launch_bounds__(64) void k_bstep_dt(BatchArgs B, int nb, int preamb
   const int rec = (st->increment % st->freq) == 0;
   compute_time_step(st, rec, crit);
   init_step(st);
+  if (st->status)
+  {
+    // an element error in this increment's pass: the increment counts and
+    // nothing after it runs, as the reference stops inside
+    // computeInternalForces (no nodal update, no endStep)
+    st->active = 0;
+    B.t1[bar] = wall_clock64();
+    return;
+  }
   st->active = 1;
   if (!end_step(st)) B.t1[bar] = wall_clock64();
 }

@@ -103,6 +103,16 @@ __global__ __launch_bounds__(256) void k_batch_quad4(BatchArgs B)
   if (threadIdx.x == 0) B.t0[bar] = wall_clock64();
   // solve() preamble: computeInternalForces, computeTimeStep(True)
   PASS::run(a, a32, e0, e1, Ls);
+  // a bar stops at its first element error (status is set by atomics at L2):
+  // the failing increment counts and nothing after it runs, as the reference
+  // stops inside computeInternalForces
+  if (threadIdx.x == 0) s_go = __hip_atomic_load(&st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+  __syncthreads();
+  if (!s_go)
+  {
+    if (threadIdx.x == 0) B.t1[bar] = wall_clock64();
+    return;
+  }
   double crit = bar_courant(a, e0, e1, red);
   if (threadIdx.x == 0) compute_time_step(st, 1, crit);
   __syncthreads();
@@ -115,9 +125,7 @@ __global__ __launch_bounds__(256) void k_batch_quad4(BatchArgs B)
       // decision uses the increment count as it stands here, before initStep
       if (threadIdx.x == 0)
       {
-        // a bar stops at its first element error (status is set by atomics at L2)
-        const int err = __hip_atomic_load(&st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_go = end_step(st) && err == 0;
+        s_go = end_step(st);
         s_rec = (st->increment % st->freq) == 0;
       }
       __syncthreads();
@@ -133,6 +141,9 @@ __global__ __launch_bounds__(256) void k_batch_quad4(BatchArgs B)
     if (threadIdx.x == 0) init_step(st);
     __syncthreads();
     PASS::run(a, a32, e0, e1, Ls);
+    if (threadIdx.x == 0) s_go = __hip_atomic_load(&st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    __syncthreads();
+    if (!s_go) break;
     bar_nodal<FE32>(B.na, n0, n1, st);
     __syncthreads();
   }

@@ -554,14 +554,26 @@ static int run_solve(dnlg_batch *b, double up_time, long long max_steps, int eng
   b->args.max_steps = max_steps;
   BHIP(hipMemsetAsync(b->t0, 0, sizeof(unsigned long long) * b->nb, b->stream));
   BHIP(hipMemsetAsync(b->t1, 0, sizeof(unsigned long long) * b->nb, b->stream));
-  hipEvent_t e0, e1;
-  BHIP(hipEventCreate(&e0));
-  BHIP(hipEventCreate(&e1));
-  BHIP(hipEventRecord(e0, b->stream));
+  // the events, and the engine code that compact_tables reads (no graph for
+  // stepped-plain), are put back on every way out
+  struct Scope
+  {
+    dnlg_batch *b;
+    int engine;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Scope()
+    {
+      b->engine = engine;
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } sc{b, b->engine};
+  BHIP(hipEventCreate(&sc.e0));
+  BHIP(hipEventCreate(&sc.e1));
+  BHIP(hipEventRecord(sc.e0, b->stream));
   long long issued = 0;
   const bool stepped = engine != DNLG_BATCH_PERSISTENT;
-  const int saved_engine = b->engine;
-  b->engine = engine;  // compact_tables reads it (no graph for stepped-plain)
+  b->engine = engine;
   if (!stepped)
   {
     if (b->form == DNLG_FORM_FP64) launch_batch_f64(b->args, b->nb, b->stream);
@@ -570,17 +582,13 @@ static int run_solve(dnlg_batch *b, double up_time, long long max_steps, int eng
   }
   else
   {
-    int rc = solve_stepped(b, max_steps, st, &issued);
-    b->engine = saved_engine;
+    const int rc = solve_stepped(b, max_steps, st, &issued);
     if (rc) return rc;
   }
-  b->engine = saved_engine;
-  BHIP(hipEventRecord(e1, b->stream));
+  BHIP(hipEventRecord(sc.e1, b->stream));
   BHIP(hipStreamSynchronize(b->stream));
   float ms = 0.f;
-  BHIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  BHIP(hipEventElapsedTime(&ms, sc.e0, sc.e1));
   if (kernel_ms) *kernel_ms = ms;
   if (wall_us)
   {
@@ -682,21 +690,24 @@ static int autotune(dnlg_batch *b, double up_time)
   if (rc) return rc;
   const int cand_engine[3] = {DNLG_BATCH_STEPPED, DNLG_BATCH_STEPPED_WIDE, DNLG_BATCH_PERSISTENT};
   int best = 0;
-  for (int c = 0; c < 3; c++)
+  bool failed = false;
+  for (int c = 0; c < 3 && !failed; c++)
   {
     double ms = 0.0;
-    rc = run_solve(b, up_time, kProbeSteps, cand_engine[c], nullptr, &ms);
-    // a probe ends on its increment budget: that status is part of the probe only
-    if (rc && rc != DNLG_E_STATE) break;
+    const int prc = run_solve(b, up_time, kProbeSteps, cand_engine[c], nullptr, &ms);
+    // a probe ends on its increment budget: that status is part of the probe
+    // only.  Another error (an element error within the first increments)
+    // ends the tuning; the solve then meets it with the stepped engine and
+    // reports it as an untuned solve would.
+    failed = prc && prc != DNLG_E_STATE;
     rc = snapshot_restore(b, sn);
     if (rc) break;
     b->probe_ms[c] = ms;
-    if (ms < b->probe_ms[best]) best = c;
+    if (!failed && ms < b->probe_ms[best]) best = c;
   }
-  const int rc2 = rc ? rc : snapshot_restore(b, sn);
   snapshot_free(sn);
-  if (rc2) return rc2;
-  b->auto_engine = cand_engine[best];
+  if (rc) return rc;
+  b->auto_engine = failed ? DNLG_BATCH_STEPPED : cand_engine[best];
   b->tuned = true;
   b->err.clear();
   return DNLG_OK;

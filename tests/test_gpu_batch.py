@@ -174,3 +174,29 @@ def test_batch_budget_and_idle_solve(workload):
     b.solve(0.0)
     assert b.time(0)["increment"] == 0 and np.array_equal(b.coords(0), meshes[0]["x"])
     b.close()
+
+
+def test_batch_element_error_same_under_autotune():
+    """a bar that inverts in its first increments: every engine stops it at the
+    same point (the failing increment counted, no nodal update after it) and
+    reports the same error; the autotuned solve, whose probes meet the error
+    first and are undone, too; the other bar still runs to the end"""
+    import redynela_amd as R
+    say("case: batch of 2, one bar at 20 km/s (inverts at once), stepped / persistent / auto")
+    good = R.mesh.taylor_axi(5, 25, 3.2e-3, 32.4e-3, 200.0)
+    bad = R.mesh.taylor_axi(5, 25, 3.2e-3, 32.4e-3, 2.0e4)
+    mat = dict(E=115e9, nu=0.3, rho=8930.0, A=400e6, B=100e6, n=1.0)
+    out = []
+    for eng in ("stepped", "persistent", "auto"):
+        b = R.Batch([good, bad], [mat, mat], solver())
+        b.set_engine(eng)
+        with pytest.raises(R.DnlgError) as ei:
+            b.solve(T_END)
+        say("  %s: %s" % (eng, ei.value))
+        out.append((str(ei.value), b.time(0)["increment"], b.time(1)["increment"], b.coords(0), b.coords(1)))
+        b.close()
+    assert "domain 1" in out[0][0] and out[0][2] < 48 < out[0][1]
+    for other in out[1:]:
+        assert other[0] == out[0][0]
+        assert other[1:3] == out[0][1:3]
+        assert np.array_equal(other[3], out[0][3]) and np.array_equal(other[4], out[0][4])
