@@ -119,3 +119,41 @@ int main(void) {
           "dnlg_solver_params.form": _lib.SolverParams.form.offset, "dnlg_mesh.coords": _lib.Mesh.coords.offset}
     for k, v in py.items():
         assert int(out[k]) == v, (k, out[k], v)
+
+
+def test_batch_create_rejects_bad_arguments_without_touching_the_gpu():
+    """dnlg_batch_create validates every domain before any device call; the
+    other batch entry points reject a null handle"""
+    from redynela_amd import _lib
+    L = _lib.lib()
+    b = C.c_void_p()
+    x = np.array([[0, 0, 0], [1e-3, 0, 0], [1e-3, 1e-3, 0], [0, 1e-3, 0]], dtype=np.float64)
+    conn = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    good = _lib.Mesh(2, 4, 1, x.ctypes.data, conn.ctypes.data)
+    mat = _lib.Material(115e9, 0.3, 8930.0, 4e8, 1e8, 1.0, 1)
+    sp = _lib.SolverParams(1.0, 0.9, 10, 1.0, 0)
+
+    def create(meshes, mats, solver, n=None):
+        ma = (_lib.Mesh * len(meshes))(*meshes)
+        mt = (_lib.Material * len(mats))(*mats)
+        return L.dnlg_batch_create(len(meshes) if n is None else n, ma, mt, None, C.byref(solver), C.byref(b))
+
+    assert create([good], [mat], sp, n=0) == 1
+    assert b"bad arguments" in L.dnlg_batch_last_error(None)
+    assert create([good], [mat], _lib.SolverParams(1.0, 0.9, 10, 1.0, 7)) == 1
+    hexmesh = _lib.Mesh(3, 4, 1, x.ctypes.data, conn.ctypes.data)
+    assert create([good, hexmesh], [mat, mat], sp) == 1
+    assert b"axisymmetric quad4" in L.dnlg_batch_last_error(None)
+    other_e = _lib.Material(200e9, 0.3, 8930.0, 4e8, 1e8, 1.0, 1)
+    assert create([good, good], [mat, other_e], sp) == 1
+    assert b"share E, nu and density" in L.dnlg_batch_last_error(None)
+    bad_conn = np.array([[0, 1, 2, 4]], dtype=np.int32)
+    outside = _lib.Mesh(2, 4, 1, x.ctypes.data, bad_conn.ctypes.data)
+    assert create([good, outside], [mat, mat], sp) == 1
+    assert b"outside its domain" in L.dnlg_batch_last_error(None)
+    assert create([good], [mat], _lib.SolverParams(1.2, 0.9, 10, 1.0, 0)) == 1
+    assert b"dissipation" in L.dnlg_batch_last_error(None)
+    assert not b.value
+    assert L.dnlg_batch_set_engine(None, 0) == 1
+    assert L.dnlg_batch_get_tuning(None, None, None) == 1
+    assert L.dnlg_batch_solve(None, 1e-6, 10, None, None) == 1
