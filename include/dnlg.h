@@ -33,6 +33,13 @@ extern "C" {
 #define DNLG_E_HIP 5            /* HIP runtime error                                               */
 #define DNLG_E_STATE 6          /* call out of sequence                                            */
 #define DNLG_E_PULSE_NOCONV 7   /* power iteration > 1000 iterations, FE/Domain.C:936-939          */
+/* Element errors (codes 2-4) are fatal in the reference (BT/Error.C:141-160).
+ * Here the handle stops at the element pass that finds one: no nodal update,
+ * endStep or later increment runs, and every later call reports the error
+ * again.  The failing increment counts; a 2D negative Jacobian, which the
+ * reference finds at the end of the previous increment, therefore leaves the
+ * increment count one above the reference's, with time and nodal state
+ * equal. */
 
 /* element families: the reference's ElQua4nAx (FE/ElQua4nAx.C) and ElHex8n3D (FE/ElHex8n3D.C) */
 #define DNLG_DIM_AXI 2

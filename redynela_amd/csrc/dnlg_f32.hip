@@ -6,7 +6,7 @@ namespace dnl {
 
 __global__ __launch_bounds__(128) void k_elem_hex8_f32(ElemArgs32 a)
 {
-  if (a.check_running && !a.st->running) return;
+  if ((a.check_running && !a.st->running) || a.st->halted) return;
   const long long e = (long long)blockIdx.x * 128 + threadIdx.x;
   if (e >= a.ne) return;
   elem_hex8_f32_body(a, e);
@@ -14,7 +14,7 @@ __global__ __launch_bounds__(128) void k_elem_hex8_f32(ElemArgs32 a)
 
 __global__ __launch_bounds__(128) void k_elem_quad4ax_f32(ElemArgs32 a)
 {
-  if (a.check_running && !a.st->running) return;
+  if ((a.check_running && !a.st->running) || a.st->halted) return;
   const long long e = (long long)blockIdx.x * 128 + threadIdx.x;
   if (e >= a.ne) return;
   elem_quad4ax_f32_body(a, e);

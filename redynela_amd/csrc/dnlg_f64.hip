@@ -29,7 +29,7 @@ constexpr int kHexNT = 256;
 template <bool SPEED_BC, bool TRACK, bool JC>
 __global__ __launch_bounds__(kHexNT) void k_elem_hex8(ElemArgs a)
 {
-  if (a.check_running && !a.st->running) return;
+  if ((a.check_running && !a.st->running) || a.st->halted) return;
   __shared__ HexLds<kHexNT> Ls;
   hex_fill_table(a, Ls);
   __syncthreads();
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kHexNT) void k_elem_hex8(ElemArgs a)
 template <bool SPEED_BC, bool TRACK, bool JC>
 __global__ __launch_bounds__(64) void k_elem_quad4ax(ElemArgs a)
 {
-  if (a.check_running && !a.st->running) return;
+  if ((a.check_running && !a.st->running) || a.st->halted) return;
   __shared__ QuadLds<64> Ls;
   quad_fill_table(a, Ls);
   __syncthreads();
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
   StepState *st = n.st;
   __shared__ double sx[768], sv[768], sa[768], sd[768], su[768];
   __shared__ double red[256];
-  if (!st->running) return;
+  if (!st->running || st->status) return;  // no nodal update after an element error
   const long long i0 = (long long)blockIdx.x * 256;
   const long long i = i0 + threadIdx.x;
   const int t3 = 3 * threadIdx.x;
@@ -276,8 +276,12 @@ __global__ __launch_bounds__(256) void k_dt(StepState *st, const double *part, i
   const bool forced = mode == 2 || mode == 4 || mode == 5;
   if (threadIdx.x == 0)
   {
-    run0 = st->running;
+    // after an element error nothing more happens: the failing increment
+    // counts, and its endStep never runs (the reference stops inside
+    // computeInternalForces)
+    run0 = st->running && !st->status;
     pend0 = !forced && st->pending;
+    if (st->status) st->halted = 1;
   }
   __syncthreads();
   if (!run0) return;

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void k_hist_pre(HistArgs h)
 {
 #pragma clang fp contract(off)
   StepState *st = h.st;
-  if (!st->running) return;
+  if (!st->running || st->status) return;
   const double tm = st->t + st->dt;  // the endStep update of this increment
   for (int f = threadIdx.x; f < h.nfiles; f += 256)
   {
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_hist_post(HistArgs h)
 {
 #pragma clang fp contract(off)
   StepState *st = h.st;
-  if (!st->running) return;
+  if (!st->running || st->status) return;
   for (int k = threadIdx.x; k < h.nitems; k += 256)
   {
     const HistItem it = h.items[k];

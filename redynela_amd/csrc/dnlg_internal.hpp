@@ -34,6 +34,8 @@ struct StepState
   double pulse_f;    // Domain::getMaximumFrequency of the last Pulse computation
   double *plot;      // [plot_cap][4] ring of endStep history lines (t, dt, computed dt, KE) or null
   long long plot_n;  // lines written so far
+  int halted;        // single-domain path: an element error has been seen by k_dt; element passes skip
+  int pad_;
 };
 
 struct ElemArgs

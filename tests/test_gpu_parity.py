@@ -199,3 +199,54 @@ def test_error_negative_jacobian_reported():
     g = R.Domain(mesh, m, s)
     with pytest.raises(R.DnlgError, match="NEG_DETJ"):
         g.begin(1e-6)
+
+
+def test_element_error_mid_run_stops_like_the_oracle():
+    """a negative Jacobian in mid-run: nothing runs after the element pass
+    that finds it, so time, coordinates and velocities equal the oracle's,
+    whether the steps were issued one at a time or as one batch of 50.  The
+    reference finds a 2D negative Jacobian in computeInternalMatrices at the
+    end of the previous increment (FE/Element2D.C, after endStep); the GPU
+    finds it in the fused element pass, after the next initStep, so its
+    increment count is one higher"""
+    import redynela_amd as R
+    from redynela_amd import _lib as L
+    say("case: axi bar at 20 km/s inverts in its first increments; GPU state after the error vs the oracle")
+    mesh = R.mesh.taylor_axi(4, 20, 3.2e-3, 32.4e-3, 2.0e4)
+    mat = dict(COPPER, A=400e6, B=100e6, n=1.0)
+
+    def gpu():
+        m = R.Material(mat["E"], mat["nu"], mat["rho"], R.IsoHardElastoplastic(mat["A"], mat["B"], mat["n"]))
+        s = R.ExplicitSolverCH()
+        s.setTimeStepMethod("Courant")
+        s.setDissipation(1.0)
+        g = R.Domain(mesh, m, s)
+        g.begin(1.0)
+        return g
+
+    o = O.Model(mesh, mat, dict(rho_b=1.0), mathmode=1)
+    assert o.begin(1.0) == 1
+    k = 0
+    while True:
+        r = o.step()
+        k += 1
+        if r < 0:
+            break
+        assert r == 1 and k < 40
+    assert r == -4  # the oracle's negative-Jacobian code (FE/Element2D.C:101-109)
+    g1 = gpu()
+    with pytest.raises(R.DnlgError, match="NEG_DETJ"):
+        for _ in range(40):
+            g1.step(1)
+    g2 = gpu()
+    with pytest.raises(R.DnlgError, match="NEG_DETJ"):
+        g2.step(50)
+    to = o.time()
+    for g in (g1, g2):
+        tg = g.time()
+        assert tg["t"] == to["t"] and tg["increment"] == to["increment"] + 1, (tg, to)
+        assert np.array_equal(g.field(L.F_COORDS), o.get(O.F_X))
+        assert np.array_equal(g.field(L.F_V), o.get(O.F_V))
+    say("  error at increment %d, t = %.6e" % (to["increment"], to["t"]))
+    g1.close()
+    g2.close()
