@@ -167,12 +167,14 @@ __device__ __forceinline__ void stage_out(double *g, const double *l, long long 
     if (base + j < lim) g[base + j] = l[j];
 }
 
+// LDS: four staging rows (x, v, a, d) of 6 KB; the new increment Delta u goes
+// out through the x row once x has been written back, and the kinetic-energy
+// partials reduce in the v row: 24 KB per block, 6 blocks per CU
 template <int DIM, bool TRACK_D, bool HAS_BCS, bool FE32>
 __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
 {
   StepState *st = n.st;
-  __shared__ double sx[768], sv[768], sa[768], sd[768], su[768];
-  __shared__ double red[256];
+  __shared__ double sx[768], sv[768], sa[768], sd[TRACK_D ? 768 : 1];
   if (!st->running || st->status) return;  // no nodal update after an element error
   const long long i0 = (long long)blockIdx.x * 256;
   const long long i = i0 + threadIdx.x;
@@ -182,10 +184,10 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
   stage_in(n.a, sa, i0, n.nn);
   if (TRACK_D) stage_in(n.d, sd, i0, n.nn);
   __syncthreads();
-  double ke = 0.0;
+  double ke = 0.0, dun[3] = {0.0, 0.0, 0.0};
   if (i < n.nn)
   {
-    double ac[3], vc[3], dun[3], vn[3], an[3], m;
+    double ac[3], vc[3], vn[3], an[3], m;
 #pragma unroll
     for (int c = 0; c < 3; c++) { ac[c] = sa[t3 + c]; vc[c] = sv[t3 + c]; }
     nodal_core<DIM, HAS_BCS, FE32>(n, i, st, ac, vc, an, vn, dun, m);
@@ -194,7 +196,6 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
     {
       sa[t3 + c] = an[c];
       sv[t3 + c] = vn[c];
-      su[t3 + c] = dun[c];
       if (TRACK_D) sd[t3 + c] = sd[t3 + c] + dun[c];
       sx[t3 + c] = sx[t3 + c] + dun[c];  // lagMotion::updateNode, FE/lagMotion.C:52-57
     }
@@ -204,18 +205,21 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
   stage_out(n.x, sx, i0, n.nn);
   stage_out(n.v, sv, i0, n.nn);
   stage_out(n.a, sa, i0, n.nn);
-  stage_out(n.du, su, i0, n.nn);
   if (TRACK_D) stage_out(n.d, sd, i0, n.nn);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 3; c++) sx[t3 + c] = dun[c];
+  if (n.ke_part) sv[threadIdx.x] = ke;
+  __syncthreads();
+  stage_out(n.du, sx, i0, n.nn);
   if (n.ke_part)
   {
-    red[threadIdx.x] = ke;
-    __syncthreads();
     for (int w = 128; w > 0; w >>= 1)
     {
-      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      if (threadIdx.x < w) sv[threadIdx.x] += sv[threadIdx.x + w];
       __syncthreads();
     }
-    if (threadIdx.x == 0) n.ke_part[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) n.ke_part[blockIdx.x] = sv[0];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->pending = 1;  // endStep of this increment is due
 }
