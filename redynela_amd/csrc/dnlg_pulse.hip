@@ -15,11 +15,12 @@
 // once when the loop has stopped or converged, so the host may issue
 // iterations ahead):
 //   k_pulse_begin  reset the iteration counters; start vector when fmax0 == 0
-//   k_pulse_ke     Ke of 8 hex8 (16 quad) elements per 64-lane block, one lane
-//                  per (element, node) holding that node's rows; per point
-//                  the lanes share the columns of C B through LDS.  Ke is
-//                  stored [8 elements][column pair J/2][8 x ND rows][2] so
-//                  that a row's 16-byte pairs are adjacent across threads
+//   k_pulse_ke     Ke of 4 hex8 (16 quad) elements per 64-lane block, two
+//                  lanes (one for the quad) per (element, node), each holding
+//                  half of that node's rows; per point the lanes share the
+//                  columns of C B through LDS.  Ke is stored [8 elements]
+//                  [column pair J/2][8 x ND rows][2] so that a row's 16-byte
+//                  pairs are adjacent across threads
 //   k_pulse_mv     y = Ke x for one row per thread (dgemv order: y = 0, then
 //                  y += x_J Ke(I, J) for J in order), y into the element's
 //                  ELL slot of the node
@@ -77,16 +78,20 @@ __global__ __launch_bounds__(256) void k_pulse_begin(PulseArgs p)
   }
 }
 
-// Ke of EPB elements per 64-lane block, one lane per (element, node): lane
-// i of an element owns the DIM rows of node i.  Per point, lane i computes
-// dN of its node and the DIM columns of C B that belong to it; the block
-// shares them through LDS and every lane adds its rows of B^T C B.  The
-// matrices go to HBM for the iterations, and the first iteration's product
-// Ke x is formed here from the rows in registers.
-template <int NEN, int DIM>
+// Ke of EPB elements per 64-lane block, SPLIT lanes per (element, node):
+// lanes (i, h) of an element own the DIM rows of node i, h the h-th of SPLIT
+// equal column ranges.  Per point, the lanes of node i compute dN of the
+// node and its DIM columns of C B (the C B rows split among them); the block
+// shares them through LDS and every lane adds its part of the node's rows of
+// B^T C B.  With two lanes per hex8 node a lane holds 36 sums instead of 72,
+// which frees registers for more waves per SIMD.  The matrices go to HBM for
+// the iterations, and the first iteration's product Ke x is formed here from
+// the rows in registers: each row's sum runs over the column ranges in order,
+// handed from lane to lane.
+template <int NEN, int DIM, int SPLIT>
 struct PulseLds
 {
-  static constexpr int ND = NEN * DIM, NIP = NEN, NC = DIM == 3 ? 6 : 4, EPB = 64 / NEN;
+  static constexpr int ND = NEN * DIM, NIP = NEN, NC = DIM == 3 ? 6 : 4, EPB = 64 / (NEN * SPLIT);
   double tab[NIP * NEN * 3];               // dN0 (and N for the quad) of the reference element
   double C[NC * NC];
   double geo[EPB][NIP][DIM * DIM + 2];     // inv(J), W (with 2 pi R for the quad), R
@@ -94,15 +99,16 @@ struct PulseLds
   double loc[EPB][ND];                     // the iterate at the element's dofs
 };
 
-template <int NEN, int DIM>
+template <int NEN, int DIM, int SPLIT>
 __global__ __launch_bounds__(64) void k_pulse_ke(PulseArgs p, ElemArgs a)
 {
 #pragma clang fp contract(off)
-  using L = PulseLds<NEN, DIM>;
-  constexpr int ND = L::ND, NIP = L::NIP, NC = L::NC, EPB = L::EPB, NT8 = 8 * ND;
+  using L = PulseLds<NEN, DIM, SPLIT>;
+  constexpr int ND = L::ND, NIP = L::NIP, NC = L::NC, EPB = L::EPB, NT8 = 8 * ND, NJ = ND / SPLIT;
+  static_assert(NJ % 2 == 0, "column ranges hold whole pairs");
   __shared__ L s;
   if (!p.st->running) return;
-  const int t = threadIdx.x, el = t / NEN, i = t % NEN;
+  const int t = threadIdx.x, el = t / (NEN * SPLIT), i = (t / SPLIT) % NEN, h = t % SPLIT;
   const long long e_raw = (long long)blockIdx.x * EPB + el;
   const bool valid = e_raw < a.ne;
   const long long e = valid ? e_raw : a.ne - 1;
@@ -125,6 +131,7 @@ __global__ __launch_bounds__(64) void k_pulse_ke(PulseArgs p, ElemArgs a)
   if (t == 0) hooke<DIM>(p.E, p.nu, s.C);
   const int *cn = a.conn + e * NEN;
   // the iterate (ev *= 1/fmax, LA/Vector.C:652-659) for the fused first product
+  if (h == 0)
   {
     const double invf = p.ctl->invf;
     const long long node = cn[i];
@@ -132,7 +139,8 @@ __global__ __launch_bounds__(64) void k_pulse_ke(PulseArgs p, ElemArgs a)
     for (int c = 0; c < DIM; c++) s.loc[el][DIM * i + c] = p.val[node * DIM + c] * invf;
   }
   __syncthreads();
-  // point geometry: FE/Element3D.C:75-105, FE/Element2D.C:84-115; lane i takes point i
+  // point geometry: FE/Element3D.C:75-105, FE/Element2D.C:84-115; lane (i, 0) takes point i
+  if (h == 0)
   {
     const int pt = i;
     double *gq = s.geo[el][pt];
@@ -176,12 +184,13 @@ __global__ __launch_bounds__(64) void k_pulse_ke(PulseArgs p, ElemArgs a)
       gq[5] = R;
     }
   }
-  double k[DIM][ND];
+  double k[DIM][NJ];
 #pragma unroll
   for (int c = 0; c < DIM; c++)
 #pragma unroll
-    for (int J = 0; J < ND; J++) k[c][J] = 0.0;
+    for (int J = 0; J < NJ; J++) k[c][J] = 0.0;
   const double *C = s.C;
+  const int j0 = h * NJ;  // first column of this lane's range
   for (int pt = 0; pt < NIP; pt++)
   {
     __syncthreads();  // geo written / previous point's CB consumed
@@ -189,40 +198,43 @@ __global__ __launch_bounds__(64) void k_pulse_ke(PulseArgs p, ElemArgs a)
     const double W = gq[DIM * DIM];
     double d[DIM];
     dgemm_row<DIM>(s.tab + (pt * NEN + i) * 3, gq, d);
-    // the DIM columns of C B of node i
+    // the DIM columns of C B of node i, rows split among the node's lanes
     if constexpr (DIM == 3)
     {
       const double d0 = d[0], d1 = d[1], d2 = d[2];
 #pragma unroll
       for (int r = 0; r < 6; r++)
-      {
-        s.CB[el][r][3 * i] = (C[r * 6 + 0] * d0 + C[r * 6 + 3] * d1 + C[r * 6 + 5] * d2);
-        s.CB[el][r][3 * i + 1] = (C[r * 6 + 1] * d1 + C[r * 6 + 3] * d0 + C[r * 6 + 4] * d2);
-        s.CB[el][r][3 * i + 2] = (C[r * 6 + 2] * d2 + C[r * 6 + 4] * d1 + C[r * 6 + 5] * d0);
-      }
+        if (r % SPLIT == h)
+        {
+          s.CB[el][r][3 * i] = (C[r * 6 + 0] * d0 + C[r * 6 + 3] * d1 + C[r * 6 + 5] * d2);
+          s.CB[el][r][3 * i + 1] = (C[r * 6 + 1] * d1 + C[r * 6 + 3] * d0 + C[r * 6 + 4] * d2);
+          s.CB[el][r][3 * i + 2] = (C[r * 6 + 2] * d2 + C[r * 6 + 4] * d1 + C[r * 6 + 5] * d0);
+        }
     }
     else
     {
       const double d0 = d[0], d1 = d[1], R = gq[5], Ni = s.tab[(pt * NEN + i) * 3 + 2];
 #pragma unroll
       for (int r = 0; r < 4; r++)
-      {
-        double v = (C[r * 4 + 0] * d0 + C[r * 4 + 2] * d1);
-        v += C[r * 4 + 3] * Ni / R;
-        s.CB[el][r][2 * i] = v;
-        s.CB[el][r][2 * i + 1] = (C[r * 4 + 1] * d1 + C[r * 4 + 2] * d0);
-      }
+        if (r % SPLIT == h)
+        {
+          double v = (C[r * 4 + 0] * d0 + C[r * 4 + 2] * d1);
+          v += C[r * 4 + 3] * Ni / R;
+          s.CB[el][r][2 * i] = v;
+          s.CB[el][r][2 * i + 1] = (C[r * 4 + 1] * d1 + C[r * 4 + 2] * d0);
+        }
     }
     __syncthreads();
-    // the DIM rows of node i of B^T C B
+    // this lane's columns of the DIM rows of node i of B^T C B
     if constexpr (DIM == 3)
     {
       const double a0 = d[0], a1 = d[1], a2 = d[2];
 #pragma unroll
-      for (int J = 0; J < ND; J++)
+      for (int J = 0; J < NJ; J++)
       {
-        const double b0 = s.CB[el][0][J], b1 = s.CB[el][1][J], b2 = s.CB[el][2][J];
-        const double b3 = s.CB[el][3][J], b4 = s.CB[el][4][J], b5 = s.CB[el][5][J];
+        const int jc = j0 + J;
+        const double b0 = s.CB[el][0][jc], b1 = s.CB[el][1][jc], b2 = s.CB[el][2][jc];
+        const double b3 = s.CB[el][3][jc], b4 = s.CB[el][4][jc], b5 = s.CB[el][5][jc];
         k[0][J] += (a0 * b0 + a1 * b3 + a2 * b5) * W;
         k[1][J] += (a1 * b1 + a0 * b3 + a2 * b4) * W;
         k[2][J] += (a2 * b2 + a1 * b4 + a0 * b5) * W;
@@ -232,35 +244,46 @@ __global__ __launch_bounds__(64) void k_pulse_ke(PulseArgs p, ElemArgs a)
     {
       const double a0 = d[0], a1 = d[1], R = gq[5], Ni = s.tab[(pt * NEN + i) * 3 + 2];
 #pragma unroll
-      for (int J = 0; J < ND; J++)
+      for (int J = 0; J < NJ; J++)
       {
-        const double b0 = s.CB[el][0][J], b1 = s.CB[el][1][J], b2 = s.CB[el][2][J], b3 = s.CB[el][3][J];
+        const int jc = j0 + J;
+        const double b0 = s.CB[el][0][jc], b1 = s.CB[el][1][jc], b2 = s.CB[el][2][jc], b3 = s.CB[el][3][jc];
         k[0][J] += (a0 * b0 + a1 * b2) * W;
         k[0][J] += b3 * Ni / R * W;
         k[1][J] += (a1 * b1 + a0 * b2) * W;
       }
     }
   }
-  if (!valid) return;
   // Ke to HBM: [e / 8][column pair][8 elements x ND rows][2]
-  double2 *ko = reinterpret_cast<double2 *>(p.ke + (e / 8) * ND * NT8);
-  const int row0 = (int)(e % 8) * ND + DIM * i;
+  if (valid)
+  {
+    double2 *ko = reinterpret_cast<double2 *>(p.ke + (e / 8) * ND * NT8);
+    const int row0 = (int)(e % 8) * ND + DIM * i;
 #pragma unroll
-  for (int c = 0; c < DIM; c++)
+    for (int c = 0; c < DIM; c++)
 #pragma unroll
-    for (int J = 0; J < ND; J += 2) ko[(long long)(J / 2) * NT8 + row0 + c] = make_double2(k[c][J], k[c][J + 1]);
-  // the first iteration's product, LAPACK dgemv order (y = 0, y += x_J Ke(I, J))
+      for (int J = 0; J < NJ; J += 2)
+        ko[(long long)((j0 + J) / 2) * NT8 + row0 + c] = make_double2(k[c][J], k[c][J + 1]);
+  }
+  // the first iteration's product, LAPACK dgemv order (y = 0, y += x_J Ke(I, J)
+  // for J in order): lane (i, h) continues the sums lane (i, h - 1) left
 #pragma unroll
   for (int c = 0; c < DIM; c++)
   {
     double y = 0.0;
 #pragma unroll
-    for (int J = 0; J < ND; J++)
+    for (int r = 0; r < SPLIT; r++)
     {
-      const double tj = 1.0 * s.loc[el][J];
-      y = y + tj * k[c][J];
+      if (r > 0) y = __shfl(y, t - 1, 64);  // lane (i, r) takes the sum of lane (i, r - 1)
+      if (h == r)
+#pragma unroll
+        for (int J = 0; J < NJ; J++)
+        {
+          const double tj = 1.0 * s.loc[el][j0 + J];
+          y = y + tj * k[c][J];
+        }
     }
-    p.y[(long long)p.fpos[e * NEN + i] * DIM + c] = y;
+    if (valid && h == SPLIT - 1) p.y[(long long)p.fpos[e * NEN + i] * DIM + c] = y;
   }
 }
 
@@ -377,17 +400,17 @@ void launch_pulse_begin(const PulseArgs &p, hipStream_t s)
 
 void launch_pulse_ke(int dim, const PulseArgs &p, const ElemArgs &a, hipStream_t s)
 {
-  // 8 hex8 or 16 quad elements per 64-lane block; the first iteration's
+  // 4 hex8 (two lanes per node) or 16 quad elements per 64-lane block; the first iteration's
   // product is formed here, its node and control kernels follow
   const unsigned grid_n = (unsigned)((p.nn + 255) / 256);
   if (dim == 3)
   {
-    hipLaunchKernelGGL((k_pulse_ke<8, 3>), dim3((unsigned)((a.ne + 7) / 8)), dim3(64), 0, s, p, a);
+    hipLaunchKernelGGL((k_pulse_ke<8, 3, 2>), dim3((unsigned)((a.ne + 3) / 4)), dim3(64), 0, s, p, a);
     hipLaunchKernelGGL((k_pulse_node<3>), dim3(grid_n), dim3(256), 0, s, p);
   }
   else
   {
-    hipLaunchKernelGGL((k_pulse_ke<4, 2>), dim3((unsigned)((a.ne + 15) / 16)), dim3(64), 0, s, p, a);
+    hipLaunchKernelGGL((k_pulse_ke<4, 2, 1>), dim3((unsigned)((a.ne + 15) / 16)), dim3(64), 0, s, p, a);
     hipLaunchKernelGGL((k_pulse_node<2>), dim3(grid_n), dim3(256), 0, s, p);
   }
   hipLaunchKernelGGL(k_pulse_ctl, dim3(1), dim3(256), 0, s, p, (int)grid_n);
