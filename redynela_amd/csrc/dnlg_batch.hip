@@ -60,16 +60,70 @@ __global__ __launch_bounds__(64) void k_bstep_dt(BatchArgs B, int nb, int preamb
   if (!end_step(st)) B.t1[bar] = wall_clock64();
 }
 
+// [nn][3] rows of the block's nodes through LDS, so that global loads and
+// stores cover whole contiguous spans (the block's nodes are contiguous and
+// end at the bar's last node at most)
+template <int NT>
+__device__ __forceinline__ void bstage_in(const double *g, double *l, long long i0, long long iend)
+{
+  const long long base = 3 * i0, lim = 3 * iend;
+#pragma unroll
+  for (int j = threadIdx.x; j < 3 * NT; j += NT) l[j] = (base + j < lim) ? g[base + j] : 0.0;
+}
+template <int NT>
+__device__ __forceinline__ void bstage_out(double *g, const double *l, long long i0, long long iend)
+{
+  const long long base = 3 * i0, lim = 3 * iend;
+#pragma unroll
+  for (int j = threadIdx.x; j < 3 * NT; j += NT)
+    if (base + j < lim) g[base + j] = l[j];
+}
+
+// nodal phase of one block of a bar's nodes: the arithmetic of bar_node,
+// with x, v, a, d staged through LDS (Delta u leaves through the x rows)
 template <bool FE32, int NT>
 __global__ __launch_bounds__(NT) void k_bstep_nodal(BatchArgs B, const int2 *blk)
 {
+#pragma clang fp contract(off)
+  __shared__ double sx[3 * NT], sv[3 * NT], sa[3 * NT], sd[3 * NT];
   const int2 bb = blk[blockIdx.x];
   const StepState *st = B.states + bb.x;
   if (!st->active) return;
   const BarDesc &d = B.bars[bb.x];
-  const long long i = (long long)bb.y + threadIdx.x;
-  if (i >= d.node0 + d.nn) return;
-  bar_node<FE32>(B.na, i, st);
+  const NodalArgs &n = B.na;
+  const long long i0 = bb.y, iend = d.node0 + d.nn, i = i0 + threadIdx.x;
+  const int t3 = 3 * threadIdx.x;
+  bstage_in<NT>(n.x, sx, i0, iend);
+  bstage_in<NT>(n.v, sv, i0, iend);
+  bstage_in<NT>(n.a, sa, i0, iend);
+  bstage_in<NT>(n.d, sd, i0, iend);
+  __syncthreads();
+  double dun[3] = {0.0, 0.0, 0.0};
+  if (i < iend)
+  {
+    double ac[3], vc[3], an[3], vn[3], m;
+#pragma unroll
+    for (int c = 0; c < 3; c++) { ac[c] = sa[t3 + c]; vc[c] = sv[t3 + c]; }
+    nodal_core<2, false, FE32>(n, i, st, ac, vc, an, vn, dun, m);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+      sa[t3 + c] = an[c];
+      sv[t3 + c] = vn[c];
+      sd[t3 + c] = sd[t3 + c] + dun[c];
+      sx[t3 + c] = sx[t3 + c] + dun[c];
+    }
+  }
+  __syncthreads();
+  bstage_out<NT>(n.x, sx, i0, iend);
+  bstage_out<NT>(n.v, sv, i0, iend);
+  bstage_out<NT>(n.a, sa, i0, iend);
+  bstage_out<NT>(n.d, sd, i0, iend);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 3; c++) sx[t3 + c] = dun[c];
+  __syncthreads();
+  bstage_out<NT>(n.du, sx, i0, iend);
 }
 
 // FP64 element pass: NT / 4 elements (NT points) per block; 5 waves per SIMD
