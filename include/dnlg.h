@@ -312,7 +312,10 @@ int dnlg_batch_taylor_params(int n, unsigned long long seed, int *nr, double *v0
  * Chung-Hulbert update, the restraints, the element-order assembly, the
  * element kinematics (closed-form 2x2 polar decomposition), the radial return
  * with its elastic/plastic switch, and the time step (Courant arg-min element,
- * LINEARdbl extrapolation).  After it the handle holds the post-forward state. */
+ * LINEARdbl extrapolation).  With ckpt_every = 1 every increment's state is
+ * kept (nsteps snapshots) and the sweep reverses over them with no
+ * recomputation: memory for time, the same gradient bit for bit.  After it
+ * the handle holds the post-forward state. */
 typedef struct
 {
   double forward_ms;      /* dnlg_adjoint_forward, device time incl. checkpoint copies  */

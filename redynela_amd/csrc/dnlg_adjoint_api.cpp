@@ -129,7 +129,8 @@ extern "C" int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_e
     }                                                                               \
   } while (0)
   ACK(dalloc(&A->ckpt, (size_t)L.size * A->nseg));
-  ACK(dalloc(&A->tape, (size_t)L.size * (A->every + 1)));
+  // with a checkpoint after every increment the checkpoints are the tape
+  if (A->every > 1) ACK(dalloc(&A->tape, (size_t)L.size * (A->every + 1)));
   ACK(dalloc(&A->fin, (size_t)L.size));
   ACK(dalloc(&A->xb, h->nn * 2));
   ACK(dalloc(&A->ub, h->nn * 2));
@@ -145,8 +146,8 @@ extern "C" int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_e
   ACK(dalloc(&A->th_part, 3 * nbe));
   ACK(dalloc(&A->sc, 1));
 #undef ACK
-  A->bytes = sizeof(double) * ((size_t)L.size * (A->nseg + A->every + 2) + h->nn * 10 + 11 * nipt + h->ne +
-                               2 * nslot + nbn + 3 * nbe) + sizeof(AdjScalars);
+  A->bytes = sizeof(double) * ((size_t)L.size * (A->nseg + (A->every > 1 ? A->every + 1 : 0) + 1) + h->nn * 10 +
+                               11 * nipt + h->ne + 2 * nslot + nbn + 3 * nbe) + sizeof(AdjScalars);
   hipEvent_t e0, e1;
   DNLG_HIP(h, hipEventCreate(&e0));
   DNLG_HIP(h, hipEventCreate(&e1));
@@ -245,7 +246,30 @@ extern "C" int dnlg_adjoint_backward(dnlg_handle *h, const double *xbar, double 
   DNLG_HIP(h, hipEventCreate(&e2));
   DNLG_HIP(h, hipEventRecord(e0, s));
   float recompute_ms = 0.f;
-  for (int seg = A->nseg - 1; seg >= 0; seg--)
+  // the reverse increment k over the states after increments k and k + 1
+  auto reverse = [&](long long k, const double *t0, const double *t1) {
+    g.x = t0 + L.x;
+    g.v = t0 + L.v;
+    g.a = t0 + L.a;
+    g.du = t0 + L.du;
+    g.S = t0 + L.S;
+    g.evp = t0 + L.evp;
+    g.p = t0 + L.p;
+    g.a1 = t1 + L.a;
+    g.st0 = (const StepState *)(t0 + L.st);
+    g.st1 = (const StepState *)(t1 + L.st);
+    g.k = k;
+    launch_adjoint_step(g, s);
+  };
+  if (A->every == 1)
+  {
+    // a checkpoint after every increment: the sweep reads them directly, with
+    // no recomputation (the states are the ones a recomputed segment records)
+    for (long long k = A->nsteps - 1; k >= 0; k--)
+      reverse(k, A->ckpt + (size_t)k * L.size, k + 1 < A->nsteps ? A->ckpt + (size_t)(k + 1) * L.size : A->fin);
+    DNLG_HIP(h, hipGetLastError());
+  }
+  for (int seg = A->every == 1 ? -1 : A->nseg - 1; seg >= 0; seg--)
   {
     const long long k0 = (long long)seg * A->every, k1 = std::min<long long>(k0 + A->every, A->nsteps);
     // recompute the segment, recording every increment's state
@@ -271,19 +295,8 @@ extern "C" int dnlg_adjoint_backward(dnlg_handle *h, const double *xbar, double 
     // reverse increments k1 .. k0+1
     for (long long k = k1 - 1; k >= k0; k--)
     {
-      const double *t0 = A->tape + (size_t)(k - k0) * L.size, *t1 = t0 + L.size;
-      g.x = t0 + L.x;
-      g.v = t0 + L.v;
-      g.a = t0 + L.a;
-      g.du = t0 + L.du;
-      g.S = t0 + L.S;
-      g.evp = t0 + L.evp;
-      g.p = t0 + L.p;
-      g.a1 = t1 + L.a;
-      g.st0 = (const StepState *)(t0 + L.st);
-      g.st1 = (const StepState *)(t1 + L.st);
-      g.k = k;
-      launch_adjoint_step(g, s);
+      const double *t0 = A->tape + (size_t)(k - k0) * L.size;
+      reverse(k, t0, t0 + L.size);
     }
     DNLG_HIP(h, hipGetLastError());
     DNLG_HIP(h, hipStreamSynchronize(s));

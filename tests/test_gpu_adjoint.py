@@ -156,3 +156,27 @@ def test_torch_autograd_function():
     g = prob.backward(2 * (prob.dom.coords[:, :2] - xt.numpy()))
     assert torch.allclose(theta.grad, torch.from_numpy(g), rtol=1e-12, atol=0)
     prob.close()
+
+
+def test_adjoint_checkpoint_every_increment_needs_no_recompute():
+    """checkpoints after every increment: the sweep reads them directly, with
+    no recomputation, and gives the gradient of the segment-recomputing sweep
+    bit for bit"""
+    from redynela_amd.adjoint import HardeningAdjoint
+    import redynela_amd as R
+    say("case: axi 4x16, 150 increments: gradient with checkpoints every increment == every 40")
+    mesh = R.mesh.taylor_axi(4, 16, 3.2e-3, 5.2e-3, 200.0)
+    th = (90e6, 292e6, 0.31)
+    out = []
+    for every in (40, 1):
+        p = HardeningAdjoint(mesh, nsteps=150, ckpt_every=every)
+        x = p.forward(th)
+        g = p.backward(np.ones((x.shape[0], 2)))
+        st = p.stats
+        say("  every %d: backward %.2f ms, recompute %.2f ms, %d checkpoints" % (every, st["backward_ms"],
+                                                                             st["recompute_ms"], st["n_checkpoints"]))
+        out.append((x, g, st))
+        p.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[1][2]["recompute_ms"] == 0.0 and out[1][2]["n_checkpoints"] == 150
