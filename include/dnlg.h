@@ -174,6 +174,13 @@ long long dnlg_field_size(dnlg_handle *h, int field);
 /* time scalars: out[0] t, [1] dt, [2] increment, [3] computed time step,
  * [4] kinetic energy of the last step, [5] alphaM, [6] beta, [7] gamma, [8] OmegaS */
 int dnlg_get_time(dnlg_handle *h, double *out);
+/* Structure::readData on a restart (FE/Structure.C:915-931, driven by
+ * Sources/Applications/DynELA_solve/main.C:106-130): the current time of a
+ * handle that has not begun.  With dnlg_set_field of the coordinates, the
+ * Current nodal fields and the point state read from a result file, the next
+ * dnlg_begin continues as a restarted reference run does: mass from the
+ * restored coordinates, a fresh time-step history, increment count 0. */
+int dnlg_set_time(dnlg_handle *h, double t);
 
 /* Pulse time step (Domain::getMaximumFrequency, FE/Domain.C:816-946; no
  * accessor in the reference): out[0] calls so far, [1] iterations of the

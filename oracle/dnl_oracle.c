@@ -1432,6 +1432,35 @@ long dnlo_get(const dnlo *m, int field, double *out)
   return n;
 }
 
+/* overwrite a state field (a restart, FE/Domain.C:1378-1404: coordinates,
+ * the Current nodal fields, point state); returns the doubles written or -1 */
+long dnlo_set(dnlo *m, int field, const double *in)
+{
+  long n = 0;
+  double *dst = NULL;
+  switch (field)
+  {
+  case F_X: dst = m->x; n = m->nn * 3; break;
+  case F_V: dst = m->v[m->cur]; n = m->nn * 3; break;
+  case F_A: dst = m->a[m->cur]; n = m->nn * 3; break;
+  case F_DU: dst = m->du[m->cur]; n = m->nn * 3; break;
+  case F_D: dst = m->d[m->cur]; n = m->nn * 3; break;
+  case F_S: dst = m->S; n = m->ne * m->nip * 9; break;
+  case F_EVP: dst = m->evp; n = m->ne * m->nip; break;
+  case F_P: dst = m->p; n = m->ne * m->nip; break;
+  case F_EPS: dst = m->Eps; n = m->ne * m->nip * 9; break;
+  case F_EPSPLAS: dst = m->EpsPlas; n = m->ne * m->nip * 9; break;
+  case F_SIG: dst = m->Sig; n = m->ne * m->nip * 9; break;
+  case F_DEPS: dst = m->dEps; n = m->ne * m->nip * 9; break;
+  default: return -1;
+  }
+  memcpy(dst, in, sizeof(double) * n);
+  return n;
+}
+
+/* Structure::readData's currentTime (FE/Structure.C:915-931) */
+void dnlo_set_time(dnlo *m, double t) { m->t = t; }
+
 /* overwrite the Current displacement increment (for single-element tests) */
 void dnlo_set_du(dnlo *m, const double *du) { memcpy(m->du[m->cur], du, sizeof(double) * m->nn * 3); }
 

@@ -29,7 +29,7 @@ HF_FIELDS = {"density": (0, False), "densityInc": (1, False), "energy": (2, Fals
 EXPORTS = [
     "dnlg_abi_version", "dnlg_create", "dnlg_destroy", "dnlg_set_option", "dnlg_compute_internal_forces",
     "dnlg_begin", "dnlg_step", "dnlg_solve", "dnlg_step_async", "dnlg_synchronize", "dnlg_get_field",
-    "dnlg_set_field", "dnlg_field_size", "dnlg_get_time", "dnlg_pulse_info", "dnlg_plot_rows", "dnlg_history_file_add", "dnlg_history_file_rows", "dnlg_get_stream", "dnlg_enable_kernel_timing",
+    "dnlg_set_field", "dnlg_field_size", "dnlg_get_time", "dnlg_set_time", "dnlg_pulse_info", "dnlg_plot_rows", "dnlg_history_file_add", "dnlg_history_file_rows", "dnlg_get_stream", "dnlg_enable_kernel_timing",
     "dnlg_kernel_stats", "dnlg_last_error", "dnlg_mesh_axi_bar", "dnlg_mesh_ogrid_sizes", "dnlg_mesh_ogrid_bar",
     "dnlg_batch_create", "dnlg_batch_destroy", "dnlg_batch_solve", "dnlg_batch_get_time", "dnlg_batch_get_field",
     "dnlg_batch_last_error", "dnlg_batch_taylor_params", "dnlg_batch_set_engine", "dnlg_batch_get_tuning",
@@ -99,6 +99,7 @@ def lib():
             "dnlg_set_field": (i, [vp, i, vp, ll]),
             "dnlg_field_size": (ll, [vp, i]),
             "dnlg_get_time": (i, [vp, vp]),
+            "dnlg_set_time": (i, [vp, d]),
             "dnlg_pulse_info": (i, [vp, vp]),
             "dnlg_plot_rows": (ll, [vp, vp, ll]),
             "dnlg_history_file_add": (i, [vp, d, d, d, i, vp, vp, vp, d, C.POINTER(i)]),

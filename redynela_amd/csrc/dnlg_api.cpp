@@ -1112,6 +1112,21 @@ extern "C" int dnlg_set_field(dnlg_handle *h, int field, const double *in, long 
   return DNLG_OK;
 }
 
+extern "C" int dnlg_set_time(dnlg_handle *h, double t)
+{
+  if (!h) return DNLG_E_ARG;
+  if (h->begun || h->inc != 0)
+  {
+    h->err = "dnlg_set_time: only on a handle that has not begun (a restart)";
+    return DNLG_E_STATE;
+  }
+  StepState s;
+  DNLG_HIP(h, hipMemcpy(&s, h->st, sizeof s, hipMemcpyDeviceToHost));
+  s.t = t;
+  DNLG_HIP(h, hipMemcpy(h->st, &s, sizeof s, hipMemcpyHostToDevice));
+  return DNLG_OK;
+}
+
 extern "C" int dnlg_get_time(dnlg_handle *h, double *out)
 {
   if (!h || !out) return DNLG_E_ARG;

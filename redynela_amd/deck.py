@@ -693,7 +693,7 @@ class Deck:
         base, ext = os.path.splitext(self.source)
         return base if ext == ".dyn" else self.source
 
-    def run(self, form="fp64", on_save=None, results=None):
+    def run(self, form="fp64", on_save=None, results=None, restart=None):
         """Structure::solve on the GPU (FP64 by default): the save-time loop of
         FE/Structure.C:809-897 with saveResults' test (Structure.C:786-799)
         and startSaveTime defaulting to saveTime (Structure.C:364-366).
@@ -702,6 +702,10 @@ class Deck:
         reference's layout (redynela_amd.results), which needs the strain
         fields and so runs with strain tracking; it also writes the domain's
         `<base>_defaultDomain.plot` history (redynela_amd.results.PlotFile).
+        restart=N continues from result file N of <base>, as DynELA_solve's
+        restart option does (Sources/Applications/DynELA_solve/main.C:106-130):
+        the state is read back (redynela_amd.results.restore), the next save
+        is at (N + 1) saveTime and the files go on from N + 1.
         Returns the Domain."""
         from . import domain as D
         from . import results as RF
@@ -739,6 +743,13 @@ class Deck:
             where = os.path.dirname(results) if results is not None else os.path.dirname(self.source or "")
             hist.append((hf, RF.HistoryFileWriter(os.path.join(where, hfo.fileName), hf.columns)))
         save = self.save if self.save is not None else (0.0, sol.stop, sol.stop)
+        resume = None
+        if restart is not None:
+            if writer is None:
+                raise DeckError("restart needs the result files' base name (results=)")
+            t0 = RF.restore(dom, "%s.%03d.bin" % (results, int(restart)))
+            writer.count = int(restart) + 1
+            resume = (t0, (int(restart) + 1) * save[2])
 
         def solve(up_time):
             dom.solve(up_time)
@@ -754,11 +765,11 @@ class Deck:
             if on_save is not None:
                 on_save(t, dom)
 
-        structure_solve(sol.stop, save, solve, saved)
+        structure_solve(sol.stop, save, solve, saved, resume=resume)
         return dom
 
 
-def structure_solve(end_time, save_times, solve, save):
+def structure_solve(end_time, save_times, solve, save, resume=None):
     """The one-domain loop of Structure::solve (FE/Structure.C:833-883).
 
     save_times = (start, stop, every) as Structure::setSaveTimes takes them,
@@ -766,9 +777,12 @@ def structure_solve(end_time, save_times, solve, save):
     advances the domain to t and returns its current time; save(t) is
     called where saveResults writes a file, under its test
     currentTime >= nextSaveTime and currentTime != previousSaveTime
-    (FE/Structure.C:786-799).  Returns the list of save times."""
+    (FE/Structure.C:786-799).  resume = (current time, next save time) of a
+    restarted run.  Returns the list of save times."""
     start, _, every = save_times
     st = dict(next=start if start != 0 else every, prev=-1.0, cur=0.0)
+    if resume is not None:
+        st["cur"], st["next"] = resume
     times = []
 
     def save_results():

@@ -252,6 +252,10 @@ class Domain:
         _lib.check(L.dnlg_get_field(self.h, f, out.ctypes.data, n), self.h)
         return out
 
+    def set_time(self, t):
+        """Structure::readData's currentTime on a restart (before the first solve)."""
+        _lib.check(_lib.lib().dnlg_set_time(self.h, float(t)), self.h)
+
     def set_field(self, f, values):
         a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
         _lib.check(_lib.lib().dnlg_set_field(self.h, f, a.ctypes.data, a.size), self.h)

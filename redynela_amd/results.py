@@ -122,6 +122,33 @@ def read(path, n_nodes, n_ips):
     return dict(time=raw[6], domain_time=raw[13], nodes=nodes, ips=ips)
 
 
+def restore(dom, path):
+    """Structure::readData of a restart (FE/Structure.C:915-931,
+    FE/Domain.C:1378-1404, as Sources/Applications/DynELA_solve/main.C:106-130
+    drives it) into a GPU Domain (FP64 form) that has not begun: coordinates,
+    the Current nodal fields (v, a, Delta u, d), the point state (p, evp, S,
+    and Eps, EpsPlas when strains are tracked) and the current time.  The next
+    solve recomputes the mass from the restored coordinates and starts a
+    fresh time-step history with increment 0, as the restarted reference run
+    does.  Returns the file's time."""
+    from . import _lib as L
+    nn = dom.n_nodes
+    nip = 8 if dom.dim == 3 else 4
+    nipt = dom.field(L.F_EVP).size
+    rec = read(path, nn, nipt)
+    nodes, ips = rec["nodes"], rec["ips"]
+    for f, c in ((L.F_COORDS, 0), (L.F_V, 8), (L.F_A, 11), (L.F_DU, 17), (L.F_D, 20)):
+        dom.set_field(f, nodes[:, c:c + 3])
+    dom.set_field(L.F_EVP, ips[:, 1])
+    dom.set_field(L.F_S, ips[:, 2:11].T)
+    dom.set_field(L.F_P, ips[:, 0].reshape(-1, nip)[:, 0])  # equal at the points of an element
+    if L.lib().dnlg_field_size(dom.h, L.F_EPS) > 0:
+        dom.set_field(L.F_EPS, ips[:, 20:29].T)
+        dom.set_field(L.F_EPSPLAS, ips[:, 38:47].T)
+    dom.set_time(rec["time"])
+    return rec["time"]
+
+
 def _tensors(dom, f, nipt):
     return dom.field(f).reshape(9, nipt).T
 

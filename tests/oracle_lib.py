@@ -25,6 +25,10 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
         L = C.CDLL(LIB_PATH)
+        L.dnlo_set.restype = C.c_long
+        L.dnlo_set.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.dnlo_set_time.restype = None
+        L.dnlo_set_time.argtypes = [C.c_void_p, C.c_double]
         L.dnlo_det_log.restype = C.c_double
         L.dnlo_det_log.argtypes = [C.c_double]
         L.dnlo_det_exp.restype = C.c_double
@@ -190,6 +194,15 @@ class Model:
 
     def compute_internal_forces(self):
         return lib().dnlo_compute_internal_forces(self.h)
+
+    def set(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        n = lib().dnlo_set(self.h, field, v)
+        if n != v.size:
+            raise RuntimeError("oracle set: field %d takes %d doubles, got %d" % (field, n, v.size))
+
+    def set_time(self, t):
+        lib().dnlo_set_time(self.h, float(t))
 
     def set_du(self, du):
         lib().dnlo_set_du(self.h, np.ascontiguousarray(du, dtype=np.float64).reshape(-1))

@@ -238,3 +238,60 @@ def test_deck_mixed_form_runs():
     assert err < 1e-5, err
     a.close()
     b.close()
+
+
+def test_deck_restart_matches_oracle_restart(tmp_path):
+    """DynELA_solve's restart (Sources/Applications/DynELA_solve/main.C:106-130):
+    the axisymmetric deck is run, then restarted from its 1st result file; the
+    files it writes from there on equal, byte for byte, those of the oracle
+    restarted from the same file (state read back, mass from the restored
+    coordinates, fresh time-step history), and they differ from the files of
+    the uninterrupted run, as the reference's restart is no exact continuation"""
+    from redynela_amd import deck
+    from redynela_amd import results as RF
+    say("case: axi deck restarted from result file 1, GPU against the oracle")
+    d = deck.load(os.path.join(HERE, "decks", "taylor_axi_small.dyn"))
+    _, mat, sol = d.model()
+    base = str(tmp_path / "run" / "axi")
+    d.run(results=base).close()
+    full = sorted(f for f in os.listdir(tmp_path / "run") if f.endswith(".bin"))
+    assert len(full) >= 3
+    # GPU restart from file 1 into a new base: files 2.. are written
+    rbase = str(tmp_path / "restart" / "axi")
+    os.makedirs(tmp_path / "restart")
+    import shutil
+    shutil.copy("%s.001.bin" % base, "%s.001.bin" % rbase)
+    dom = d.run(results=rbase, restart=1)
+    dom.close()
+    # the oracle, restarted from the same file
+    o = O.from_deck(d)
+    nn, nipt = o.nn, o.get(O.F_EVP).size
+    rec = RF.read("%s.001.bin" % base, nn, nipt)
+    nodes, ips = rec["nodes"], rec["ips"]
+    for f, c in ((O.F_X, 0), (O.F_V, 8), (O.F_A, 11), (O.F_DU, 17), (O.F_D, 20)):
+        o.set(f, nodes[:, c:c + 3])
+    o.set(O.F_P, ips[:, 0])
+    o.set(O.F_EVP, ips[:, 1])
+    o.set(O.F_S, ips[:, 2:11])
+    o.set(O.F_EPS, ips[:, 20:29])
+    o.set(O.F_EPSPLAS, ips[:, 38:47])
+    o.set_time(rec["time"])
+    paths = []
+
+    def save(t):
+        _, nb, ib = O.result_blocks(o, mat.density, mat.init_temp)
+        paths.append(str(tmp_path / ("oracle.%03d.bin" % (2 + len(paths)))))
+        RF.write(paths[-1], t, nb, ib)
+
+    def solve(t):
+        assert o.solve(t) >= 0
+        return o.time()["t"]
+
+    save_times = d.save if d.save is not None else (0.0, sol.stop, sol.stop)
+    deck.structure_solve(sol.stop, save_times, solve, save, resume=(rec["time"], 2 * save_times[2]))
+    gpu = sorted(f for f in os.listdir(tmp_path / "restart") if f.endswith(".bin"))[1:]
+    assert len(gpu) == len(paths) == len(full) - 1
+    for g, r in zip(gpu, paths):
+        assert open(tmp_path / "restart" / g, "rb").read() == open(r, "rb").read(), g
+    # not the uninterrupted run's files (fresh mass and time-step history)
+    assert open(tmp_path / "restart" / gpu[-1], "rb").read() != open(tmp_path / "run" / full[-1], "rb").read()
