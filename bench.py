@@ -23,7 +23,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# algorithmic HBM bytes per element-step (FP64 hex8), from the survey's breakdown:
+# algorithmic HBM bytes per element-step (FP64 hex8), the minimum the algorithm moves:
 #   integration-point history S(6)+p+evp = 8 words x 8 B x 2 (read+write) x 8 points = 1024 B
 #   connectivity 8 x 4 B = 32 B
 #   nodal traffic 276 B per node x (nodes / elements)
