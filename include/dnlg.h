@@ -153,6 +153,25 @@ int dnlg_set_option(dnlg_handle *h, int option, int value);
  * fint_out ([n_nodes][dim], node-major as the reference's Fint) may be NULL. */
 int dnlg_compute_internal_forces(dnlg_handle *h, double *fint_out);
 
+/* Domain::computeInternalForces() as ExplicitSolverCH::initIteration calls it
+ * inside the reference's own time loop (FE/ExplicitSolverCH.C:471-481, after
+ * a host computePredictions): the caller's node coordinates (Node::coords)
+ * and Current displacement increment (Current->delta_disp), [n_nodes][3]
+ * host arrays, replace the handle's device copies before the pass (NULL keeps
+ * the device copy).  The integration-point state stays in device memory, as
+ * it stays in the reference's IntegrationPoints between calls.  fint_out as
+ * above. */
+int dnlg_compute_internal_forces_from(dnlg_handle *h, const double *coords, const double *delta_disp,
+                                      double *fint_out);
+
+/* Domain::computeCourantTimeStep() (FE/Domain.h:128, FE/Domain.C:1613-1646)
+ * on the handle's present coordinates: crit = min over elements of
+ * getLength() / getSoundSpeed(), the value Solver::computeTimeStep turns into
+ * fmax = 2 / crit (FE/Solver.C:228-231).  Hexahedra use the build-defined
+ * length V / A_max (the reference's ElHex8n3D::getLength exits). */
+int dnlg_courant_time_step(dnlg_handle *h, double *crit);
+
+
 /* ExplicitSolverCH::solve(upTime) (FE/ExplicitSolverCH.C:115-278) in parts:
  * dnlg_begin runs the solve() preamble (mass matrix once, internal forces,
  * forced time step); dnlg_step runs up to nsteps passes of the time loop and

@@ -1340,31 +1340,58 @@ int dnlo_init(dnlo *m)
   return m->status;
 }
 
-/* ExplicitSolverCH::solve preamble, FE/ExplicitSolverCH.C:115-172 */
-int dnlo_begin(dnlo *m, double upTime)
+/* The time loop in parts, so that a caller can supply Domain::Fint from
+ * outside (the drop-in test: the reference's host loop with the GPU element
+ * pass in place of Domain::computeInternalForces).  Each *_pre runs up to the
+ * computeInternalForces call of the reference and each *_post from there on;
+ * fint = NULL takes the oracle's own Fint. */
+
+/* ExplicitSolverCH::solve preamble up to computeInternalForces,
+ * FE/ExplicitSolverCH.C:115-157.  Returns 1 when the loop runs, 0 when
+ * betweenTimeBounds is false, <0 on error. */
+int dnlo_begin_pre(dnlo *m, double upTime)
 {
   m->upTime = upTime;
   if (!((m->t >= 0.0) && (m->t < (m->endTime < upTime ? m->endTime : upTime)))) return 0;
   int st = dnlo_compute_internal_matrices(m);
   if (st) return -st;
   o_mass(m);
-  st = dnlo_compute_internal_forces(m);
-  if (st) return -st;
-  st = o_compute_time_step(m, 1);
+  return 1;
+}
+
+/* the preamble's computeTimeStep(True), FE/ExplicitSolverCH.C:172 */
+int dnlo_begin_post(dnlo *m, const double *fint)
+{
+  if (fint) memcpy(m->fint, fint, sizeof(double) * m->nn * m->dim);
+  int st = o_compute_time_step(m, 1);
   if (st) return -st;
   return 1;
 }
 
-/* one pass of the while(runStep) loop, FE/ExplicitSolverCH.C:179-274.
- * Returns 1 while the loop continues, 0 when it ends, <0 on error. */
-int dnlo_step(dnlo *m)
+/* ExplicitSolverCH::solve preamble, FE/ExplicitSolverCH.C:115-172 */
+int dnlo_begin(dnlo *m, double upTime)
 {
-  /* initStep, FE/ExplicitSolverCH.C:434-447 */
+  int r = dnlo_begin_pre(m, upTime);
+  if (r <= 0) return r;
+  int st = dnlo_compute_internal_forces(m);
+  if (st) return -st;
+  return dnlo_begin_post(m, NULL);
+}
+
+/* initStep and computePredictions, FE/ExplicitSolverCH.C:181-212, 387-447 */
+void dnlo_step_pre(dnlo *m)
+{
   if (m->t + m->dt > m->upTime) m->dt = m->upTime - m->t;
   m->increment++;
   o_predictions(m);
-  int st = dnlo_compute_internal_forces(m);
-  if (st) return -st;
+}
+
+/* explicitSolve, updateGrid, endStep and, while the loop runs,
+ * computeInternalMatrices + computeTimeStep, FE/ExplicitSolverCH.C:222-274.
+ * Returns 1 while the loop continues, 0 when it ends, <0 on error. */
+int dnlo_step_post(dnlo *m, const double *fint)
+{
+  if (fint) memcpy(m->fint, fint, sizeof(double) * m->nn * m->dim);
   o_explicit_solve(m);
   /* endStep, FE/ExplicitSolverCH.C:450-468 */
   m->t += m->dt;
@@ -1377,12 +1404,22 @@ int dnlo_step(dnlo *m)
   int run = (m->t >= 0.0) && (m->t < (m->endTime < m->upTime ? m->endTime : m->upTime));
   if (run)
   {
-    st = dnlo_compute_internal_matrices(m);
+    int st = dnlo_compute_internal_matrices(m);
     if (st) return -st;
     st = o_compute_time_step(m, 0);
     if (st) return -st;
   }
   return run;
+}
+
+/* one pass of the while(runStep) loop, FE/ExplicitSolverCH.C:179-274.
+ * Returns 1 while the loop continues, 0 when it ends, <0 on error. */
+int dnlo_step(dnlo *m)
+{
+  dnlo_step_pre(m);
+  int st = dnlo_compute_internal_forces(m);
+  if (st) return -st;
+  return dnlo_step_post(m, NULL);
 }
 
 /* ExplicitSolverCH::solve(upTime) in full; returns the number of steps or <0 */

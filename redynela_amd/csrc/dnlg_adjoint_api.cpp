@@ -95,6 +95,14 @@ extern "C" int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_e
     h->err = "dnlg_adjoint_forward: needs a handle that has not stepped yet";
     return DNLG_E_STATE;
   }
+  // the reverse sweep takes every one of the nsteps increments as a full
+  // Courant / LINEARdbl step: an end time that stops the loop early would
+  // leave no-op increments behind it
+  if (s0.endTime < 1e300)
+  {
+    h->err = "dnlg_adjoint_forward: the handle's end_time must be unbounded (1e300); run the increments wanted with nsteps";
+    return DNLG_E_ARG;
+  }
   adj_free(h->adj);
   h->adj = nullptr;
   AdjStore *A = new AdjStore();
@@ -168,6 +176,13 @@ extern "C" int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_e
   DNLG_HIP(h, hipEventRecord(e1, h->stream));
   rc = dnlg_synchronize(h);
   if (rc) return rc;
+  StepState s1;
+  DNLG_HIP(h, hipMemcpy(&s1, h->st, sizeof s1, hipMemcpyDeviceToHost));
+  if (s1.increment != nsteps || !s1.running)
+  {
+    h->err = "dnlg_adjoint_forward: the solve loop did not run all nsteps increments";
+    return DNLG_E_STATE;
+  }
   DNLG_HIP(h, hipEventElapsedTime(&A->fwd_ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -30,7 +30,8 @@ void launch_gather_fint(int dim, const unsigned char *deg, const double *fe, con
                         long long nn, hipStream_t s);
 void launch_elem_f32(int dim, const ElemArgs32 &a, hipStream_t s);
 void launch_nodal_f64(int dim, const NodalArgs &n, bool track_d, bool has_bcs, hipStream_t s);
-void launch_courant_f64(int dim, const ElemArgs &a, double *part, hipStream_t s);
+void launch_courant_f64(int dim, const ElemArgs &a, double cs, double *part, hipStream_t s);
+void launch_min_reduce(const double *part, int npart, double *out, hipStream_t s);
 void launch_dt(StepState *st, const double *part, int npart, int recompute, const double *ke_part, int nke, int mode,
                hipStream_t s);
 unsigned nodal_blocks(long long nn);
@@ -167,7 +168,8 @@ static void free_all(dnlg_handle *h)
   void *ptrs[] = {h->x, h->v, h->a, h->du, h->d, h->mass, h->fint, h->fe, h->S, h->evp, h->p, h->eps, h->epsplas, h->deps,
                   h->me, h->part, h->ke_part, h->bcs_v, h->conn, h->fpos, h->deg, h->bc, h->bcs, h->st, h->T,
                   h->S32, h->evp32, h->T32, h->p32, h->fe32, h->pulse_ke, h->pulse_val, h->pulse_init,
-                  h->pulse_part, h->pulse_ctl, h->pulse_y == h->fe ? nullptr : h->pulse_y, h->plot_ring};
+                  h->pulse_part, h->pulse_ctl, h->pulse_y == h->fe ? nullptr : h->pulse_y, h->plot_ring,
+                  h->crit};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -319,6 +321,7 @@ extern "C" int dnlg_create(const dnlg_mesh *mesh, const dnlg_material *mat, cons
   }
   CK(dalloc(&h->me, (long long)h->maxdeg * nn));
   CK(dalloc(&h->part, courant_blocks(ne)));
+  CK(dalloc(&h->crit, 1));
   CK(dalloc(&h->ke_part, nodal_blocks(nn)));
   CK(dalloc(&h->conn, ne * nen));
   CK(dalloc(&h->fpos, ne * nen));
@@ -362,6 +365,7 @@ extern "C" int dnlg_create(const dnlg_mesh *mesh, const dnlg_material *mat, cons
     ro /= nen;
     s.cs = sqrt((E * (1 - Nu)) / (ro * (1 + Nu) * (1 - 2 * Nu)));
   }
+  h->cs = s.cs;
   CK(hipMemcpy(h->st, &s, sizeof s, hipMemcpyHostToDevice));
 #undef CK
   fill_elem_args(h);
@@ -669,6 +673,37 @@ extern "C" int dnlg_compute_internal_forces(dnlg_handle *h, double *fint_out)
   return check_status(h, s);
 }
 
+// Domain::computeInternalForces() inside the caller's own time loop: the
+// caller's coordinates and Current displacement increment replace the
+// device copies, then the element pass and the Fint gather run as above
+extern "C" int dnlg_compute_internal_forces_from(dnlg_handle *h, const double *coords, const double *delta_disp,
+                                                 double *fint_out)
+{
+  if (!h) return DNLG_E_ARG;
+  int rc = finalize(h);
+  if (rc) return rc;
+  const size_t nb = sizeof(double) * (size_t)h->nn * 3;
+  if (coords) DNLG_HIP(h, hipMemcpyAsync(h->x, coords, nb, hipMemcpyHostToDevice, h->stream));
+  if (delta_disp) DNLG_HIP(h, hipMemcpyAsync(h->du, delta_disp, nb, hipMemcpyHostToDevice, h->stream));
+  return dnlg_compute_internal_forces(h, fint_out);
+}
+
+// Domain::computeCourantTimeStep (FE/Domain.C:1613-1646) on the handle's
+// present coordinates: per-block minima of l / c into the handle's buffer,
+// then their minimum
+extern "C" int dnlg_courant_time_step(dnlg_handle *h, double *crit_out)
+{
+  if (!h || !crit_out) return DNLG_E_ARG;
+  int rc = finalize(h);
+  if (rc) return rc;
+  launch_courant_f64(h->dim, h->ea, h->cs, h->part, h->stream);
+  launch_min_reduce(h->part, (int)courant_blocks(h->ne), h->crit, h->stream);
+  DNLG_HIP(h, hipGetLastError());
+  DNLG_HIP(h, hipMemcpyAsync(crit_out, h->crit, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  DNLG_HIP(h, hipStreamSynchronize(h->stream));
+  return DNLG_OK;
+}
+
 // close the last issued increment: endStep (+ computeTimeStep while running)
 static int finalize(dnlg_handle *h)
 {
@@ -685,7 +720,7 @@ static int finalize(dnlg_handle *h)
   }
   else
   {
-    if (rec) launch_courant_f64(h->dim, h->ea, h->part, h->stream);
+    if (rec) launch_courant_f64(h->dim, h->ea, h->cs, h->part, h->stream);
     launch_dt(h->st, h->part, (int)courant_blocks(h->ne), rec, h->ke_on ? h->ke_part : nullptr,
               (int)nodal_blocks(h->nn), 1, h->stream);
   }
@@ -732,7 +767,7 @@ extern "C" int dnlg_begin(dnlg_handle *h, double up_time, int *running)
   }
   else
   {
-    launch_courant_f64(h->dim, h->ea, h->part, h->stream);
+    launch_courant_f64(h->dim, h->ea, h->cs, h->part, h->stream);
     launch_dt(h->st, h->part, (int)courant_blocks(h->ne), 1, nullptr, 0, 2, h->stream);
   }
   DNLG_HIP(h, hipGetLastError());
@@ -772,7 +807,7 @@ static int issue_steps(dnlg_handle *h, int nsteps)
     }
     else
     {
-      if (rec) launch_courant_f64(h->dim, h->ea, h->part, h->stream);
+      if (rec) launch_courant_f64(h->dim, h->ea, h->cs, h->part, h->stream);
       launch_dt(h->st, h->part, npart, rec, h->ke_on ? h->ke_part : nullptr, nke, 0, h->stream);
     }
     if (ha.nfiles) launch_hist_pre(ha, h->stream);

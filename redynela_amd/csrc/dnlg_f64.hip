@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
 // Courant critical time per element, min-reduced per block
 // ---------------------------------------------------------------------------
 template <int DIM>
-__global__ __launch_bounds__(256) void k_courant(ElemArgs a, double *part)
+__global__ __launch_bounds__(256) void k_courant(ElemArgs a, double cs, double *part)
 {
   __shared__ double red[256];
   const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -240,12 +240,12 @@ __global__ __launch_bounds__(256) void k_courant(ElemArgs a, double *part)
       int nd[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) nd[k] = a.conn[e * 8 + k];
-      t = hex_length(a, nd) / (a.st->cs);
+      t = hex_length(a, nd) / (cs);
     }
     else
     {
       const double len = quad_length(a.x, a.conn + e * 4);
-      t = len / (a.st->cs);
+      t = len / (cs);
     }
   }
   red[threadIdx.x] = t;
@@ -256,6 +256,24 @@ __global__ __launch_bounds__(256) void k_courant(ElemArgs a, double *part)
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// the minimum of the first npart block minima (one block): the crit that
+// Domain::computeCourantTimeStep returns (FE/Domain.C:1613-1646); min is
+// exact, so the order of the reduction does not change the result
+__global__ __launch_bounds__(256) void k_min_reduce(const double *part, int npart, double *out)
+{
+  __shared__ double red[256];
+  double c = __builtin_inf();
+  for (int k = threadIdx.x; k < npart; k += 256) c = fmin(c, part[k]);
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1)
+  {
+    if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -429,11 +447,16 @@ void launch_nodal_f64(int dim, const NodalArgs &n, bool track_d, bool has_bcs, h
 #undef DNL_NODAL
 }
 
-void launch_courant_f64(int dim, const ElemArgs &a, double *part, hipStream_t s)
+void launch_courant_f64(int dim, const ElemArgs &a, double cs, double *part, hipStream_t s)
 {
   const unsigned grid = courant_blocks(a.ne);
-  if (dim == 3) hipLaunchKernelGGL(k_courant<3>, dim3(grid), dim3(256), 0, s, a, part);
-  else hipLaunchKernelGGL(k_courant<2>, dim3(grid), dim3(256), 0, s, a, part);
+  if (dim == 3) hipLaunchKernelGGL(k_courant<3>, dim3(grid), dim3(256), 0, s, a, cs, part);
+  else hipLaunchKernelGGL(k_courant<2>, dim3(grid), dim3(256), 0, s, a, cs, part);
+}
+
+void launch_min_reduce(const double *part, int npart, double *out, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(256), 0, s, part, npart, out);
 }
 
 void launch_dt(StepState *st, const double *part, int npart, int recompute, const double *ke_part, int nke, int mode,

@@ -24,6 +24,8 @@ struct dnlg_handle
   double *fint = nullptr, *fe = nullptr, *S = nullptr, *evp = nullptr, *p = nullptr;
   double *eps = nullptr, *epsplas = nullptr, *deps = nullptr, *me = nullptr, *part = nullptr, *ke_part = nullptr;
   double *bcs_v = nullptr;
+  double *crit = nullptr;  // Domain::computeCourantTimeStep result of dnlg_courant_time_step
+  double cs = 0.0;         // Element::getSoundSpeed of the material (the Courant kernel's argument)
   int *conn = nullptr, *fpos = nullptr;
   unsigned char *deg = nullptr;
   int maxdeg = 0;

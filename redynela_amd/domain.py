@@ -206,11 +206,28 @@ class Domain:
             self.set_option(_lib.OPT_TIME_STEP_METHOD, _lib.TS_PULSE)
 
     # -- reference interface -------------------------------------------------
-    def computeInternalForces(self):
-        """Domain::computeInternalForces(): stress update + Fint assembly."""
+    def computeInternalForces(self, coords=None, delta_disp=None):
+        """Domain::computeInternalForces(): stress update + Fint assembly.
+
+        With coords and/or delta_disp ([n_nodes, 3]) the caller's node
+        coordinates and Current displacement increment replace the device
+        copies first, as when the caller runs the time loop itself
+        (FE/ExplicitSolverCH.C:471-481)."""
         out = np.zeros((self.n_nodes, self.dim))
-        _lib.check(_lib.lib().dnlg_compute_internal_forces(self.h, out.ctypes.data), self.h)
+        L = _lib.lib()
+        if coords is None and delta_disp is None:
+            _lib.check(L.dnlg_compute_internal_forces(self.h, out.ctypes.data), self.h)
+            return out
+        x = None if coords is None else np.ascontiguousarray(coords, dtype=np.float64).reshape(self.n_nodes, 3)
+        u = None if delta_disp is None else np.ascontiguousarray(delta_disp, dtype=np.float64).reshape(self.n_nodes, 3)
+        _lib.check(L.dnlg_compute_internal_forces_from(self.h, _lib.ptr(x), _lib.ptr(u), out.ctypes.data), self.h)
         return out
+
+    def computeCourantTimeStep(self):
+        """Domain::computeCourantTimeStep(): min over elements of l / c (FE/Domain.C:1613-1646)."""
+        c = C.c_double()
+        _lib.check(_lib.lib().dnlg_courant_time_step(self.h, C.byref(c)), self.h)
+        return c.value
 
     @property
     def Fint(self):

@@ -64,6 +64,14 @@ def lib():
         L.dnlo_begin.argtypes = [C.c_void_p, C.c_double]
         L.dnlo_step.restype = C.c_int
         L.dnlo_step.argtypes = [C.c_void_p]
+        L.dnlo_begin_pre.restype = C.c_int
+        L.dnlo_begin_pre.argtypes = [C.c_void_p, C.c_double]
+        L.dnlo_begin_post.restype = C.c_int
+        L.dnlo_begin_post.argtypes = [C.c_void_p, C.c_void_p]
+        L.dnlo_step_pre.restype = None
+        L.dnlo_step_pre.argtypes = [C.c_void_p]
+        L.dnlo_step_post.restype = C.c_int
+        L.dnlo_step_post.argtypes = [C.c_void_p, C.c_void_p]
         L.dnlo_solve.restype = C.c_long
         L.dnlo_solve.argtypes = [C.c_void_p, C.c_double]
         L.dnlo_compute_internal_forces.restype = C.c_int
@@ -163,6 +171,21 @@ class Model:
     def solve(self, up_time):
         return lib().dnlo_solve(self.h, up_time)
 
+    # the loop in parts around Domain::computeInternalForces (fint=None: the oracle's own)
+    def begin_pre(self, up_time):
+        return lib().dnlo_begin_pre(self.h, up_time)
+
+    def begin_post(self, fint=None):
+        f = None if fint is None else np.ascontiguousarray(fint, dtype=np.float64)
+        return lib().dnlo_begin_post(self.h, None if f is None else f.ctypes.data)
+
+    def step_pre(self):
+        lib().dnlo_step_pre(self.h)
+
+    def step_post(self, fint=None):
+        f = None if fint is None else np.ascontiguousarray(fint, dtype=np.float64)
+        return lib().dnlo_step_post(self.h, None if f is None else f.ctypes.data)
+
     def plot_row(self):
         """(t, dt, computed dt, KE) as endStep writes them to the .plot file"""
         out = np.zeros(4)
@@ -194,6 +217,10 @@ class Model:
 
     def compute_internal_forces(self):
         return lib().dnlo_compute_internal_forces(self.h)
+
+    def courant_crit(self):
+        """Domain::computeCourantTimeStep on the present coordinates (hex8: needs computeInternalMatrices)"""
+        return lib().dnlo_courant_crit(self.h)
 
     def set(self, field, values):
         v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
