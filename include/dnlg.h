@@ -171,6 +171,21 @@ int dnlg_compute_internal_forces_from(dnlg_handle *h, const double *coords, cons
  * length V / A_max (the reference's ElHex8n3D::getLength exits). */
 int dnlg_courant_time_step(dnlg_handle *h, double *crit);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the crit of
+ * dnlg_courant_time_step, bit for bit, for a mesh in caller-owned DEVICE
+ * memory -- coords [n_elems' nodes][3] doubles, conn [n_elems][4 or 8] ints,
+ * crit one double written on the device -- with no handle.  Only young,
+ * poisson and density of *mat (a host struct) are read.  The kernels run on
+ * `stream` (a hipStream_t of the current device, NULL for the null stream)
+ * and the call returns without waiting.  It allocates nothing: the per-block
+ * minima go to a 512 KiB array that each GPU holds for the life of the
+ * process, so n_elems is at most 16,777,216 (DNLG_E_ARG above that).  Calls
+ * that overlap on one GPU share that array and race.  Errors:
+ * dnlg_noalloc_last_error(). */
+int dnlg_courant_time_step_noalloc(int dim, long long n_elems, const double *coords, const int *conn,
+                                   const dnlg_material *mat, double *crit, void *stream);
+const char *dnlg_noalloc_last_error(void);
 
 /* ExplicitSolverCH::solve(upTime) (FE/ExplicitSolverCH.C:115-278) in parts:
  * dnlg_begin runs the solve() preamble (mass matrix once, internal forces,

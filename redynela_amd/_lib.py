@@ -34,6 +34,7 @@ EXPORTS = [
     "dnlg_batch_create", "dnlg_batch_destroy", "dnlg_batch_solve", "dnlg_batch_get_time", "dnlg_batch_get_field",
     "dnlg_batch_last_error", "dnlg_batch_taylor_params", "dnlg_batch_set_engine", "dnlg_batch_get_tuning",
     "dnlg_adjoint_forward", "dnlg_adjoint_backward", "dnlg_compute_internal_forces_from", "dnlg_courant_time_step",
+    "dnlg_courant_time_step_noalloc", "dnlg_noalloc_last_error",
 ]
 BATCH_AUTO, BATCH_PERSISTENT, BATCH_STEPPED, BATCH_STEPPED_PLAIN, BATCH_STEPPED_WIDE = 0, 1, 2, 3, 4
 
@@ -124,6 +125,8 @@ def lib():
             "dnlg_adjoint_backward": (i, [vp, vp, vp, C.POINTER(AdjointStats)]),
             "dnlg_compute_internal_forces_from": (i, [vp, vp, vp, vp]),
             "dnlg_courant_time_step": (i, [vp, C.POINTER(d)]),
+            "dnlg_courant_time_step_noalloc": (i, [i, ll, vp, vp, C.POINTER(Material), vp, vp]),
+            "dnlg_noalloc_last_error": (C.c_char_p, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

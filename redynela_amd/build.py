@@ -30,6 +30,7 @@ UNITS = [
     ("dnlg_batch_api.cpp", ["-ffp-contract=off", "-x", "hip"]),
     ("dnlg_adjoint.hip", ["-ffp-contract=fast-honor-pragmas"]),
     ("dnlg_adjoint_api.cpp", ["-ffp-contract=off", "-x", "hip"]),
+    ("dnlg_noalloc.hip", ["-ffp-contract=off"]),
 ]
 
 
