@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--no-shard", action="store_true")
     ap.add_argument("--no-pulse", action="store_true")
     ap.add_argument("--no-copy-peak", action="store_true")
+    ap.add_argument("--no-full-state", action="store_true")
     ap.add_argument("--precision-report", action="store_true",
                     help="only the O-grid Johnson-Cook bar (4,000 elements per section) in both forms")
     ap.add_argument("--precision-layers", type=int, default=1000)
@@ -99,6 +100,7 @@ def main():
     dom.enable_kernel_timing(False)
     elapsed = t1 - t0
     tinfo = dom.time()
+    fp64_state = (dom.coords, dom.evp) if not args.no_fp32 else None
     value = ne * args.steps / elapsed
     avg_el = ms_el / max(n_el, 1) * 1e-3
     avg_nd = ms_nd / max(n_nd, 1) * 1e-3
@@ -110,7 +112,10 @@ def main():
     dom.close()
     fp32 = None
     if not args.no_fp32:
-        fp32 = mixed_form(mesh, args, value)
+        fp32 = mixed_form(mesh, args, value, fp64_state)
+    full_state = None
+    if not args.no_full_state:
+        full_state = full_state_leg(mesh, args)
     batched = None
     if not args.no_batch:
         batched = {form: batched_bars(form, engine=args.batch_engine) for form in ("fp64", "mixed")}
@@ -127,8 +132,11 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args)
     copy = None if args.no_copy_peak else copy_peak()
-    traffic = profiled("traffic_bytes_per_launch")
-    valu_insts = profiled("valu_insts_per_launch")
+    # per-launch counters of the element kernel from the committed profile of
+    # the same window (warm-up and timed increments); none for another window
+    window = "%d+%d" % (args.warmup, args.steps)
+    traffic = profiled("traffic_bytes_per_launch", window)
+    valu_insts = profiled("valu_insts_per_launch", window)
 
     out = {
         "metric": "element-steps/s, 1M hex8 FP64 vs FP32 (err vs FP64 oracle); batched bars/s",
@@ -150,6 +158,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_elem_hex8 (element stress/force pass)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic, "avg_ms": avg_el * 1e3, "launches": n_el,
+                     "window": "increments %d..%d after the preamble (warm-up %d, timed %d)"
+                               % (args.warmup + 1, args.warmup + args.steps, args.warmup, args.steps),
+                     "traffic_source": None if traffic is None else
+                     "profiles/k_elem_hex8_fp64_1M.md, FETCH_SIZE x 2.0 + WRITE_SIZE averaged over the same window",
                      "algorithmic_bytes_per_launch": elem_bytes, "copy_peak": copy},
         # the bound that applies to the bit-exact FP64 kernel: VALU issue, one
         # wave64 instruction per 4 cycles on each of 1,024 SIMDs at 2.4 GHz
@@ -159,6 +171,7 @@ def main():
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (elapsed / args.steps) / 1e9,
                           "nodal_kernel_avg_ms": avg_nd * 1e3},
         "fp32_form": fp32,
+        "full_state": full_state,
         "batched": batched,
         "adjoint": adjoint,
         "shard": shard,
@@ -168,8 +181,20 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def mixed_form(mesh, args, fp64_value):
-    """The mixed-precision form (FP32 element pass, FP64 nodal state) on the same workload."""
+def shape(mesh, x, evp):
+    """final bar length (max z), mushroom radius (max r on the z = 0 layer) and peak plastic strain"""
+    import numpy as np
+    base = mesh["x"][:, 2] == 0.0
+    return {"length": float(x[:, 2].max()), "radius": float(np.hypot(x[base, 0], x[base, 1]).max()),
+            "max_evp": float(evp.max())}
+
+
+def mixed_form(mesh, args, fp64_value, fp64_state=None):
+    """The mixed-precision form (FP32 element pass, FP64 nodal state) on the
+    same workload, and its error against the FP64 run at the end of the same
+    increments: bar length, mushroom radius, peak plastic strain and the
+    displacement field (max |du| / max |u|)."""
+    import numpy as np
     import redynela_amd as R
     mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
     sol = R.ExplicitSolverCH()
@@ -188,10 +213,47 @@ def mixed_form(mesh, args, fp64_value):
     el = time.perf_counter() - t0
     ne = mesh["conn"].shape[0]
     v = ne * args.steps / el
+    out = {"value": v, "unit": "element-steps/s", "ms_per_step": el / args.steps * 1e3,
+           "speedup_vs_fp64": v / fp64_value,
+           "note": "same mesh, steps and warm-up; FP32 kinematics / stress / B^T sigma, FP64 nodal state and time"}
+    if fp64_state is not None:
+        x64, e64 = fp64_state
+        x32, e32 = dom.coords, dom.evp
+        a, b = shape(mesh, x64, e64), shape(mesh, x32, e32)
+        u64, u32 = x64 - mesh["x"], x32 - mesh["x"]
+        out["err_vs_fp64"] = {k: abs(b[k] - a[k]) / abs(a[k]) for k in a}
+        out["err_vs_fp64"]["displacement"] = float(np.abs(u32 - u64).max() / np.abs(u64).max())
+        out["err_vs_fp64"]["at"] = "end of the timed increments (%d after the preamble)" % (args.warmup + args.steps)
     dom.close()
-    return {"value": v, "unit": "element-steps/s", "ms_per_step": el / args.steps * 1e3,
-            "speedup_vs_fp64": v / fp64_value,
-            "note": "same mesh, steps and warm-up; FP32 kinematics / stress / B^T sigma, FP64 nodal state and time"}
+    return out
+
+
+def full_state_leg(mesh, args):
+    """The FP64 form with the diagnostic point tensors the reference always
+    updates (Eps, EpsPlas, dEps, FE/Element.C:362-391) kept: same mesh and
+    window as the headline, which leaves them out (they never feed forces)."""
+    import redynela_amd as R
+    from redynela_amd import _lib as L
+    mat = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))
+    sol = R.ExplicitSolverCH()
+    sol.setTimeStepMethod("Courant")
+    sol.setDissipation(1.0)
+    dom = R.Domain(mesh, mat, sol, track_strain=True)
+    dom.set_option(L.OPT_STORE_FINT, 0)
+    assert dom.begin(1.0)
+    dom.step_async(args.warmup)
+    dom.synchronize()
+    dom.enable_kernel_timing(True)
+    t0 = time.perf_counter()
+    dom.step_async(args.steps)
+    dom.synchronize()
+    el = time.perf_counter() - t0
+    n_el, ms_el = dom.kernel_stats(0)
+    dom.close()
+    ne = mesh["conn"].shape[0]
+    return {"value": ne * args.steps / el, "unit": "element-steps/s", "ms_per_step": el / args.steps * 1e3,
+            "element_kernel_ms": ms_el / max(n_el, 1),
+            "note": "FP64 with Eps, EpsPlas and dEps per point also read and written (+1,728 B per element-step)"}
 
 
 def batched_bars(form, n=512, t_end=40e-6, engine="auto"):
@@ -381,89 +443,53 @@ def precision_report(layers=1000, t_end=60e-6):
     return out
 
 
-def profiled(key):
+def profiled(key, window):
     """A per-launch figure of the element kernel from the committed rocprofv3
-    summary: traffic_bytes_per_launch (FETCH_SIZE / WRITE_SIZE, calibrated
-    against a copy kernel) or valu_insts_per_launch (SQ_INSTS_VALU)."""
+    summary, for the bench window `warmup+steps`: traffic_bytes_per_launch
+    (FETCH_SIZE / WRITE_SIZE, calibrated against a copy kernel) or
+    valu_insts_per_launch (SQ_INSTS_VALU); None when that window has not been
+    profiled."""
     p = os.path.join(ROOT, "profiles", "k_elem_hex8_fp64_1M.md")
+    tag = "%s@%s:" % (key, window)
     try:
         for line in open(p):
-            if line.startswith(key + ":"):
+            if line.startswith(tag):
                 return float(line.split(":")[1])
     except OSError:
         pass
     return None
 
 
-def copy_peak():
-    """Device-to-device hipMemcpy of 2 GiB (read + write counted), best of 10,
-    timed with HIP events: the HBM bandwidth a plain streaming copy reaches
-    on this box.  Called through ctypes on the HIP runtime."""
-    import ctypes as C
-    # the HIP runtime instance libdnlgpu already uses (a second copy would see no device)
-    from redynela_amd import _lib
-    _lib.lib()
-    names = []
-    with open("/proc/self/maps") as f:
-        for ln in f:
-            path = ln.split()[-1] if ln.strip() else ""
-            if "libamdhip64.so" in path and path not in names:
-                names.append(path)
-    hip = None
-    for name in names + ["libamdhip64.so"]:
-        try:
-            cand = C.CDLL(name)
-        except OSError:
-            continue
-        cnt = C.c_int(0)
-        if cand.hipGetDeviceCount(C.byref(cnt)) == 0 and cnt.value > 0:
-            hip = cand
-            break
-    if hip is None:
-        return {"error": "no HIP runtime with a device among %s" % names}
-    vp = C.c_void_p
-    hip.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
-    hip.hipMemcpyAsync.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
-    hip.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
-    hip.hipMemset.argtypes = [vp, C.c_int, C.c_size_t]
-    hip.hipEventCreate.argtypes = [C.POINTER(vp)]
-    hip.hipEventRecord.argtypes = [vp, vp]
-    hip.hipEventSynchronize.argtypes = [vp]
-    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
-    hip.hipEventDestroy.argtypes = [vp]
-    hip.hipFree.argtypes = [vp]
-    n = 1 << 31
-    a, b = vp(), vp()
-    e0, e1 = vp(), vp()
-    rc = hip.hipMalloc(C.byref(a), n) or hip.hipMalloc(C.byref(b), n)
-    if rc:
-        return {"error": "hipMalloc returned %d" % rc}
+def cpu_model():
+    """the host CPU's model name and its logical CPU count"""
+    name = "unknown"
     try:
-        hip.hipEventCreate(C.byref(e0))
-        hip.hipEventCreate(C.byref(e1))
-        hip.hipMemset(a, 1, n)
-        hip.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
-        hip.hipDeviceSynchronize()
-        best = 0.0
-        for _ in range(10):
-            hip.hipEventRecord(e0, None)
-            hip.hipMemcpyAsync(b, a, n, 3, None)
-            hip.hipEventRecord(e1, None)
-            hip.hipEventSynchronize(e1)
-            ms = C.c_float()
-            hip.hipEventElapsedTime(C.byref(ms), e0, e1)
-            if ms.value > 0:
-                best = max(best, 2.0 * n / (ms.value * 1e-3) / 1e9)
-        return {"GB_s": best, "bytes_per_copy": 2 * n, "how": "hipMemcpyAsync device to device, 2 GiB"}
-    finally:
-        if e0.value:
-            hip.hipEventDestroy(e0)
-        if e1.value:
-            hip.hipEventDestroy(e1)
-        if a.value:
-            hip.hipFree(a)
-        if b.value:
-            hip.hipFree(b)
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return name, os.cpu_count() or 0
+
+
+def copy_peak():
+    """A plain device copy of 1 GiB (tests/capi/hbm_copy, one kernel, read +
+    write bytes counted), best of 10, timed with HIP events in a child
+    process: the HBM bandwidth a streaming copy reaches on this box."""
+    import subprocess
+    prog = os.path.join(ROOT, "tests", "capi", "hbm_copy")
+    if not os.path.exists(prog):
+        return {"error": "tests/capi/hbm_copy is not built"}
+    try:
+        r = subprocess.run([prog, "1024", "10"], capture_output=True, text=True, timeout=120)
+    except subprocess.TimeoutExpired:
+        return {"error": "hbm_copy timed out"}
+    if r.returncode != 0:
+        return {"error": "hbm_copy exited with %d: %s" % (r.returncode, r.stderr.strip()[-300:])}
+    c = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"GB_s": c["best_GB_s"], "mean_GB_s": c["mean_GB_s"], "bytes_per_copy": c["bytes_read"] + c["bytes_written"],
+            "how": "tests/capi/hbm_copy: one 16-byte-per-lane copy kernel, 1 GiB read + 1 GiB written"}
 
 
 def cpu_baseline(mesh, args):
@@ -483,7 +509,9 @@ def cpu_baseline(mesh, args):
         m.step()
     dt = time.perf_counter() - t0
     ne = mesh["conn"].shape[0]
+    model, ncpu = cpu_model()
     return {"value": ne * args.cpu_steps / dt, "unit": "element-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "logical_cpus": ncpu,
             "sample": "%d increments of the same %d-element mesh after 1 warm-up increment (oracle, OpenMP element "
                       "loop, sequential element-order assembly), %.1f s" % (args.cpu_steps, ne, dt)}
 

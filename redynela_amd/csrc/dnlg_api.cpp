@@ -650,7 +650,9 @@ static hipEvent_t next_event(dnlg_handle *h)
   if (h->ev_used >= h->ev_pool.size())
   {
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    // timing events only: no system-scope fence at record, which would write
+    // back the caches between the kernels they bracket and slow the next one
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     h->ev_pool.push_back(e);
   }
   return h->ev_pool[h->ev_used++];

@@ -188,9 +188,19 @@ __device__ __forceinline__ bool stress_update_jc(const ElemArgs &a, const T9 &dE
 template <int NT>
 struct HexLds
 {
-  double tab[8 * 8 * 3];  // dN0(ip, k, d)
-  double x[19][NT + 1];   // rows 0..8: J^-1 of the point; 9+c: Sig; 18: w detJ
+  double tab[8 * 8 * 3 + 4];  // dN0(ip, k, d) at hex_tab(ip, k) + d
+  double x[19][NT + 1];       // rows 0..8: J^-1 of the point; 9+c: Sig (9..14 first carry the element's
+                              // node coordinates and increments); 18: w detJ; columns hex_col(lane)
 };
+// LDS banks: one double spans two of the 64 four-byte banks, so doubles i and
+// i + 32 collide.  A hex8 exchange read has 8 distinct addresses per wave, one
+// per element (lanes 8m + q, m = 0..7, same q); element m + 4 would land on
+// element m's banks, so the column of lane t has bit 2 flipped when bit 5 of
+// t is set (a permutation within each wave: its writes stay conflict-free).
+// The table is read with 8 distinct Gauss points pt (phase 1) or 8 distinct
+// nodes (phase 2); 4 doubles of padding after point 3 separate pt from pt + 4.
+__device__ __forceinline__ int hex_col(int t) { return t ^ ((t >> 3) & 4); }
+__device__ __forceinline__ int hex_tab(int ip, int k) { return (ip * 8 + k) * 3 + (ip >= 4 ? 4 : 0); }
 template <int NT>
 struct QuadLds
 {
@@ -206,7 +216,7 @@ __device__ __forceinline__ void hex_fill_table(const ElemArgs &a, HexLds<NT> &L)
   {
     double g[3];
     hex_dn0(a, i / 24, (i / 3) % 8, g);
-    L.tab[i] = g[i % 3];
+    L.tab[hex_tab(i / 24, (i / 3) % 8) + i % 3] = g[i % 3];
   }
 }
 template <int NT>
@@ -247,19 +257,43 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
   const bool valid = e_raw < e_end;
   const long long e = valid ? e_raw : e_end - 1;
   const long long nipt = a.ne * 8, ip = e * 8 + pt;
-  const int *cn = a.conn + e * 8;
-  int nd[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) nd[k] = cn[k];
+  // lane pt of the element fetches local node pt (coordinates and Current
+  // increment) and the element's 8 lanes share them through LDS rows 9..14,
+  // which hold Sig and w detJ only later in the pass
+  const int mynode = a.conn[e * 8 + pt];
+  {
+    const double *xk = a.x + 3 * (long long)mynode;
+    const double *uk = a.du + 3 * (long long)mynode;
+    double u0 = uk[0], u1 = uk[1], u2 = uk[2];
+    if (SPEED_BC)
+    {
+      const unsigned char m = a.bcs[mynode];
+      if (m)
+      {
+        const double *sp = a.bcs_v + 3 * (long long)mynode;
+        if (m & 1) u0 = sp[0] * st->dt;
+        if (m & 2) u1 = sp[1] * st->dt;
+        if (m & 4) u2 = sp[2] * st->dt;
+      }
+    }
+    const int col = hex_col(tid);
+    Ls.x[9][col] = xk[0];
+    Ls.x[10][col] = xk[1];
+    Ls.x[11][col] = xk[2];
+    Ls.x[12][col] = u0;
+    Ls.x[13][col] = u1;
+    Ls.x[14][col] = u2;
+  }
+  wave_lds_sync();
 
   // Jacobian at this Gauss point, FE/Element3D.C:75-89
   T9 J = {{0, 0, 0, 0, 0, 0, 0, 0, 0}};
 #pragma unroll
   for (int k = 0; k < 8; k++)
   {
-    const double *xk = a.x + 3 * (long long)nd[k];
-    const double x0 = xk[0], x1 = xk[1], x2 = xk[2];
-    const double *tg = Ls.tab + (pt * 8 + k) * 3;
+    const int ck = hex_col(lbase + k);
+    const double x0 = Ls.x[9][ck], x1 = Ls.x[10][ck], x2 = Ls.x[11][ck];
+    const double *tg = Ls.tab + hex_tab(pt, k);
     const double g[3] = {tg[0], tg[1], tg[2]};
     J.v[0] += g[0] * x0; J.v[1] += g[1] * x0; J.v[2] += g[2] * x0;
     J.v[3] += g[0] * x1; J.v[4] += g[1] * x1; J.v[5] += g[2] * x1;
@@ -270,27 +304,16 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
   T9 inv;
   if (!inv3(J, inv) && valid) raise_status(st, ST_DETJ, e);
 #pragma unroll
-  for (int c = 0; c < 9; c++) Ls.x[c][tid] = inv.v[c];
+  for (int c = 0; c < 9; c++) Ls.x[c][hex_col(tid)] = inv.v[c];
 
   // F = I + sum_k dN(k,:) (x) du_k, FE/Element3D.C:182-205
   T9 F = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
 #pragma unroll
   for (int k = 0; k < 8; k++)
   {
-    const double *uk = a.du + 3 * (long long)nd[k];
-    double u0 = uk[0], u1 = uk[1], u2 = uk[2];
-    if (SPEED_BC)
-    {
-      const unsigned char m = a.bcs[nd[k]];
-      if (m)
-      {
-        const double *sp = a.bcs_v + 3 * (long long)nd[k];
-        if (m & 1) u0 = sp[0] * st->dt;
-        if (m & 2) u1 = sp[1] * st->dt;
-        if (m & 4) u2 = sp[2] * st->dt;
-      }
-    }
-    const double *tg = Ls.tab + (pt * 8 + k) * 3;
+    const int ck = hex_col(lbase + k);
+    const double u0 = Ls.x[12][ck], u1 = Ls.x[13][ck], u2 = Ls.x[14][ck];
+    const double *tg = Ls.tab + hex_tab(pt, k);
     const double g[3] = {tg[0], tg[1], tg[2]};
     double d[3];
     dgemm_row<3>(g, inv.v, d);
@@ -298,6 +321,7 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
     F.v[3] += d[0] * u1; F.v[4] += d[1] * u1; F.v[5] += d[2] * u1;
     F.v[6] += d[0] * u2; F.v[7] += d[1] * u2; F.v[8] += d[2] * u2;
   }
+  wave_lds_sync();  // every lane has read rows 9..14 before any writes Sig there
 
   T9 dEps, R;
   if (!polar_lnU(F, dEps, R) && valid) raise_status(st, ST_POLAR, e);
@@ -360,8 +384,8 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
   // the element's 8 lanes sit in one wave, whose LDS operations complete in
   // order, so a wave-level barrier orders the writes before the reads
 #pragma unroll
-  for (int c = 0; c < 9; c++) Ls.x[9 + c][tid] = Sig.v[c];
-  Ls.x[18][tid] = WxdJ;
+  for (int c = 0; c < 9; c++) Ls.x[9 + c][hex_col(tid)] = Sig.v[c];
+  Ls.x[18][hex_col(tid)] = WxdJ;
   wave_lds_sync();
 
   // nodal force of local node L = pt, accumulated point by point as
@@ -371,7 +395,7 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
 #pragma unroll
   for (int q = 0; q < 8; q++)
   {
-    const int src = lbase + q;
+    const int src = hex_col(lbase + q);
     double sq[9];
 #pragma unroll
     for (int c = 0; c < 9; c++) sq[c] = Ls.x[9 + c][src];
@@ -379,7 +403,7 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
     double iq[9];
 #pragma unroll
     for (int c = 0; c < 9; c++) iq[c] = Ls.x[c][src];
-    const double *tg = Ls.tab + (q * 8 + L) * 3;
+    const double *tg = Ls.tab + hex_tab(q, L);
     const double g[3] = {tg[0], tg[1], tg[2]};
     double d[3];
     dgemm_row<3>(g, iq, d);
