@@ -43,6 +43,25 @@
  * hex8 characteristic length (the reference's ElHex8n3D::getLength exits,
  * FE/ElHex8n3D.C:188-193) and the deterministic log/exp/pow used in
  * "det" math mode (the reference calls glibc).
+ *
+ * How each part is pinned (the tests named are under tests/):
+ *   - linear algebra (polar decomposition, 3x3 inverse, dgemm, RFI product,
+ *     deviator, J2, norm): bit for bit against the reference's own LA/LP
+ *     sources compiled under oracle/_ref (test_oracle_linalg.py);
+ *   - element layer: closed forms and invariants -- shape tables, lumped
+ *     mass, uniaxial Hencky response, radial return, rigid rotation, patch
+ *     test, axisymmetric hoop strain, Courant lengths, sum of forces
+ *     (test_oracle_element.py); the axisymmetric nodal forces of a uniform
+ *     stress in a ring element, pressure averaging under a quadratic
+ *     increment, the first time step (test_oracle_pins.py);
+ *   - Chung-Hulbert update at any rho_b: its one-increment amplification
+ *     matrix equals the published explicit generalized-alpha recurrence
+ *     (test_oracle_pins.py); central-difference identity at rho_b = 1 and the
+ *     LINEARdbl time-step extrapolation (test_oracle_run.py);
+ *   - Pulse power iteration against numpy eigenvalues (test_oracle_pulse.py).
+ *   Parity unpinned: the Newton iterate sequence for n < 1 (only its
+ *   converged consistency is checked), the BoundarySpeed rule, and glibc's
+ *   last bits of log / pow (within 1 ulp, test_oracle_math.py).
  */
 #include <math.h>
 #include <stdint.h>
