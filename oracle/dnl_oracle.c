@@ -509,6 +509,16 @@ typedef struct
   int status;
   /* time-step method: 0 Courant, 1 Pulse (Solver::setTimeStepMethod, FE/Solver.C:312-327) */
   int ts_method;
+  /* assembly strategy of Domain::computeInternalForces (FE/Domain.C:130-738):
+   * 0 the oracle's own order (element-parallel work, sequential assembly in
+   * element order = case 0 with one thread); 2, 3, 6, 7 the reference's cases
+   * (dnlo_set_strategy) */
+  int strategy;
+  int nthreads;      /* threads of the strategy (omp_get_max_threads in the reference) */
+  int mapped;        /* Jobs::mapInitial done (FE/Jobs.C:129-131: once) */
+  int *job_el;       /* elements of job 0, then job 1, ... (the sorted centroid order) */
+  long *job_off;     /* [nthreads + 1] offsets into job_el */
+  double *part_fint; /* [nthreads][nn * dim] per-thread partial Fint */
   /* Domain::getMaximumFrequency state (FE/Domain.h:65-66): the last iterate and
    * fmax0, kept from one call to the next; Ke per element [ne][nd*nd] */
   double fmax0;
@@ -921,7 +931,7 @@ int dnlo_compute_internal_matrices(dnlo *m)
  * local node by local node.  Element work runs in parallel into a per-element
  * buffer; the assembly stays sequential, so the sums are those of the
  * single-threaded reference. */
-int dnlo_compute_internal_forces(dnlo *m)
+static int o_forces_element_order(dnlo *m)
 {
   const int dim = m->dim, nen = m->nen;
 #pragma omp parallel for schedule(static)
@@ -940,6 +950,198 @@ int dnlo_compute_internal_forces(dnlo *m)
     }
   }
   return O_OK;
+}
+
+/* ---- the reference's multi-threaded assembly strategies ----
+ * Jobs::mapInitial (FE/Jobs.C:129-165): the elements sorted by
+ * compareElementJobsSort (FE/Jobs.C:104-125: centroid = sum of the node
+ * coordinates / n; an element goes after another when its centroid y is
+ * greater, or y is equal and its x is not smaller) with List::sort's Shell
+ * sort (BT/List.h:1346-1372: gaps h = 3h + 1, insertion while the comparator
+ * says "after"), then cut into equal shares: sorted element i goes to job
+ * min(floor((i + 1) / (nb / np)), np - 1).  Done once, on the coordinates of
+ * the first call, as the reference's `init` flag does. */
+static int o_jobs_after(const double *c1, const double *c2)
+{
+  if (c1[1] < c2[1]) return 0;
+  if (c1[1] > c2[1]) return 1;
+  if (c1[0] < c2[0]) return 0;
+  return 1;
+}
+
+static void o_map_initial(dnlo *m)
+{
+  const long nb = m->ne;
+  const int np = m->nthreads, nen = m->nen;
+  double *cen = o_alloc((size_t)nb * 3);
+  int *lst = (int *)malloc(sizeof(int) * (nb ? nb : 1));
+  for (long e = 0; e < nb; e++)
+  {
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < nen; i++)
+      for (int c = 0; c < 3; c++) v[c] += m->x[3 * (long)m->conn[e * nen + i] + c];
+    for (int c = 0; c < 3; c++) cen[3 * e + c] = v[c] / nen;
+    lst[e] = (int)e;
+  }
+  long h = 1;
+  while (h <= nb) h = 3 * h + 1;
+  while (h != 1)
+  {
+    h = h / 3;
+    for (long i = h + 1; i <= nb; i++)
+    {
+      const int v = lst[i - 1];
+      long j = i;
+      while (o_jobs_after(cen + 3 * (long)lst[j - h - 1], cen + 3 * (long)v))
+      {
+        lst[j - 1] = lst[j - h - 1];
+        j -= h;
+        if (j <= h) break;
+      }
+      lst[j - 1] = v;
+    }
+  }
+  const double nppr = ((double)nb) / np;
+  long *cnt = (long *)calloc(np, sizeof(long));
+  int *job = (int *)malloc(sizeof(int) * (nb ? nb : 1));
+  for (long i = 0; i < nb; i++)
+  {
+    const double q = (i + 1) / nppr;
+    job[i] = q <= np - 1 ? (int)(long)q : np - 1;
+    cnt[job[i]]++;
+  }
+  free(m->job_el);
+  free(m->job_off);
+  m->job_el = (int *)malloc(sizeof(int) * (nb ? nb : 1));
+  m->job_off = (long *)calloc(np + 1, sizeof(long));
+  for (int p = 0; p < np; p++) m->job_off[p + 1] = m->job_off[p] + cnt[p];
+  long *fill = (long *)calloc(np, sizeof(long));
+  for (long i = 0; i < nb; i++) m->job_el[m->job_off[job[i]] + fill[job[i]]++] = lst[i];
+  free(fill);
+  free(cnt);
+  free(job);
+  free(lst);
+  free(cen);
+  m->mapped = 1;
+}
+
+/* per-thread partial Fint of one element's forces: reverse local-node order
+ * as Vector::gatherFrom (LA/Vector.C:828-889, cases 5 and 6) or forward as
+ * the explicit loops of cases 2, 3, 4 and 7 */
+static void o_add_element(const dnlo *m, long e, const double *Fe, double *f, int reverse)
+{
+  const int dim = m->dim, nen = m->nen;
+  const int *cn = m->conn + e * nen;
+  for (int k = 0; k < nen; k++)
+  {
+    const int i = reverse ? nen - 1 - k : k;
+    const long glob = (long)cn[i] * dim;
+    for (int j = 0; j < dim; j++) f[glob + j] += Fe[i * dim + j];
+  }
+}
+
+static int o_forces_strategy(dnlo *m)
+{
+  const long noddl = m->nn * m->dim;
+  const int np = m->nthreads, st = m->strategy, nd = m->nen * m->dim;
+  if (st != 2 && !m->mapped) o_map_initial(m);
+  if (!m->part_fint) m->part_fint = o_alloc((size_t)np * noddl);
+  int status = O_OK;
+  long next = 0; /* case 2: the shared position in the element list */
+  if (st == 3)
+    for (long k = 0; k < noddl; k++) m->fint[k] = 0.0; /* Fint = 0.0, FE/Domain.C:146 */
+#pragma omp parallel num_threads(np) reduction(| : status)
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    double *f = m->part_fint + (size_t)t * noddl;
+    for (long k = 0; k < noddl; k++) f[k] = 0.0;
+    if (st == 2)
+    {
+      /* FE/Domain.C:241-316: each thread takes the next element of the list in a critical section */
+      for (;;)
+      {
+        long e;
+#pragma omp critical(dnlo_case2)
+        e = next < m->ne ? next++ : -1;
+        if (e < 0) break;
+        double *Fe = m->Fe + e * nd;
+        const int s = o_element_forces(m, e, Fe);
+        status |= s;
+        if (!s) o_add_element(m, e, Fe, f, 0);
+      }
+    }
+    else
+    {
+      /* cases 3, 6, 7 (FE/Domain.C:416-496, 503-594, 654-729): the thread's job in sorted order */
+      for (long k = m->job_off[t]; k < m->job_off[t + 1]; k++)
+      {
+        const long e = m->job_el[k];
+        double *Fe = m->Fe + e * nd;
+        const int s = o_element_forces(m, e, Fe);
+        status |= s;
+        if (!s) o_add_element(m, e, Fe, f, st == 6);
+      }
+    }
+    if (st == 3)
+    {
+      /* Fint += FintLocal in a critical section: the order of arrival */
+#pragma omp critical(dnlo_case3)
+      for (long k = 0; k < noddl; k++) m->fint[k] += f[k];
+    }
+  }
+  if (status) return status;
+  if (st != 3)
+  {
+    /* the fixed-order reduction, FE/Domain.C:309-316 */
+#pragma omp parallel for schedule(static) num_threads(np)
+    for (long i = 0; i < noddl; i++)
+    {
+      double val = 0;
+      for (int j = 0; j < np; j++) val += m->part_fint[(size_t)j * noddl + i];
+      m->fint[i] = val;
+    }
+  }
+  return O_OK;
+}
+
+/* Domain::computeInternalForces: the oracle's order (strategy 0) or one of
+ * the reference's multi-threaded strategies */
+int dnlo_compute_internal_forces(dnlo *m)
+{
+  if (m->strategy == 0) return o_forces_element_order(m);
+  return o_forces_strategy(m);
+}
+
+/* the strategy of Domain::computeInternalForces (the reference's environment
+ * variable computeInternalForces, FE/Domain.C:150-153): 0 = the oracle's
+ * order, or the reference's case 2, 3, 6 or 7 with `threads` threads (the
+ * deterministic ones, 6 and 7, sum the same way for a given thread count).
+ * Returns 0, or -1 for another value. */
+int dnlo_set_strategy(dnlo *m, int strategy, int threads)
+{
+  if (strategy != 0 && strategy != 2 && strategy != 3 && strategy != 6 && strategy != 7) return -1;
+  if (threads < 1) threads = 1;
+  if (threads != m->nthreads)
+  {
+    free(m->part_fint);
+    m->part_fint = NULL;
+    m->mapped = 0;
+  }
+  m->strategy = strategy;
+  m->nthreads = threads;
+  return 0;
+}
+
+/* the jobs of the last Jobs::mapInitial: element of each position, job offsets */
+long dnlo_jobs(const dnlo *m, int *elements, long *offsets)
+{
+  if (!m->mapped) return 0;
+  if (elements) memcpy(elements, m->job_el, sizeof(int) * m->ne);
+  if (offsets) memcpy(offsets, m->job_off, sizeof(long) * (m->nthreads + 1));
+  return m->nthreads;
 }
 
 /* Domain::computeMassMatrix + Element::computeMassMatrix (FE/Domain.C:948-1005,
@@ -1542,5 +1744,6 @@ void dnlo_free(dnlo *m)
   free(m->ev); free(m->Ke);
   free(m->S); free(m->Eps); free(m->EpsPlas); free(m->dEps); free(m->R); free(m->Sig);
   free(m->p); free(m->evp); free(m->T); free(m->dN); free(m->detJ); free(m->fint); free(m->Fe); free(m->Fe_status);
+  free(m->job_el); free(m->job_off); free(m->part_fint);
   free(m);
 }

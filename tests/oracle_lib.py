@@ -58,6 +58,10 @@ def lib():
         L.dnlo_set_ke.argtypes = [C.c_void_p, C.c_int]
         L.dnlo_set_speed_bc.argtypes = [C.c_void_p, C.c_long, C.c_int, _dp]
         L.dnlo_set_threads.argtypes = [C.c_int]
+        L.dnlo_set_strategy.restype = C.c_int
+        L.dnlo_set_strategy.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.dnlo_jobs.restype = C.c_long
+        L.dnlo_jobs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.dnlo_init.restype = C.c_int
         L.dnlo_init.argtypes = [C.c_void_p]
         L.dnlo_begin.restype = C.c_int
@@ -217,6 +221,21 @@ class Model:
 
     def compute_internal_forces(self):
         return lib().dnlo_compute_internal_forces(self.h)
+
+    def set_strategy(self, strategy, threads):
+        """Domain::computeInternalForces strategy: 0 (element order) or the reference's case 2, 3, 6, 7"""
+        if lib().dnlo_set_strategy(self.h, int(strategy), int(threads)) != 0:
+            raise ValueError("oracle: no strategy %r" % (strategy,))
+
+    def jobs(self):
+        """(elements in job order, job offsets) of the last Jobs::mapInitial, or None"""
+        n = lib().dnlo_jobs(self.h, None, None)
+        if n <= 0:
+            return None
+        el = np.zeros(self.ne, dtype=np.int32)
+        off = np.zeros(n + 1, dtype=np.int64)
+        lib().dnlo_jobs(self.h, el.ctypes.data, off.ctypes.data)
+        return el, off
 
     def courant_crit(self):
         """Domain::computeCourantTimeStep on the present coordinates (hex8: needs computeInternalMatrices)"""
