@@ -55,6 +55,7 @@ def main():
                     help="only the O-grid Johnson-Cook bar (4,000 elements per section) in both forms")
     ap.add_argument("--precision-layers", type=int, default=1000)
     ap.add_argument("--precision-tend", type=float, default=60e-6)
+    ap.add_argument("--precision-forms", default="fp64,mixed", help="forms of the precision report, comma-separated")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this build runs one domain on one GPU; --gpus %d is not supported "
@@ -65,8 +66,8 @@ def main():
     from redynela_amd import _lib as L
 
     if args.precision_report:
-        print(json.dumps({"precision_report": precision_report(args.precision_layers, args.precision_tend)}),
-              flush=True)
+        print(json.dumps({"precision_report": precision_report(args.precision_layers, args.precision_tend,
+                                                               args.precision_forms.split(","))}), flush=True)
         return
     if args.batch_only:
         print(json.dumps({"batched": {form: batched_bars(form, engine=args.batch_engine)
@@ -80,20 +81,14 @@ def main():
     sol.setDissipation(1.0)
     dom = R.Domain(mesh, mat, sol)
     dom.set_option(L.OPT_STORE_FINT, 0)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is optional here: dnlg_synchronize waits for the stream
-        sync = lambda: None  # noqa: E731
+    # dnlg_synchronize waits for the library's stream, the only one the steps run on
     assert dom.begin(1.0)
     dom.step_async(args.warmup)
     dom.synchronize()
-    sync()
     dom.enable_kernel_timing(True)
     t0 = time.perf_counter()
     dom.step_async(args.steps)
     dom.synchronize()
-    sync()
     t1 = time.perf_counter()
     n_el, ms_el = dom.kernel_stats(0)
     n_nd, ms_nd = dom.kernel_stats(1)
@@ -295,7 +290,17 @@ def batched_bars(form, n=512, t_end=40e-6, engine="auto"):
     nr, v0, sy0, hm = R.batch.taylor_params(n)
     slow = [{"bar": int(i), "nr": int(nr[i]), "v0": float(v0[i]), "sigma_y0": float(sy0[i]), "H": float(hm[i]),
              "increments": int(steps[i]), "wall_us": float(wall_us[i])} for i in np.argsort(-steps)[:5]]
-    return {"bars": n, "engine": engine, "autotune": tuning, "bars_per_s": n / (ms * 1e-3), "solve_ms": ms, "host_solve_s": host_s,
+    # algorithmic HBM bytes per axisymmetric element-step: point history 4 points x 8 words x 8 B x 2, connectivity
+    # 16 B and nodal traffic 188 B (FP64); 444 B in the mixed form (FP32 point history)
+    per_step = 716.0 if form == "fp64" else 444.0
+    es = float((ne * steps).sum())
+    roof = {"bound": "hbm", "algorithmic_bytes_per_element_step": per_step,
+            "achieved": es * per_step / (ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": es * per_step / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "over": "the whole batched solve (element, scalar and nodal launches), device time",
+            "counters": "profiles/batch_taylor512_counters.md"}
+    return {"bars": n, "engine": engine, "autotune": tuning, "bars_per_s": n / (ms * 1e-3), "solve_ms": ms,
+            "roofline": roof, "host_solve_s": host_s,
             "p50_us_per_bar": float(np.percentile(wall_us, 50)), "p99_us_per_bar": float(np.percentile(wall_us, 99)),
             "max_us_per_bar": float(wall_us.max()), "element_steps": int((ne * steps).sum()),
             "element_steps_per_s": float((ne * steps).sum() / (ms * 1e-3)),
@@ -402,11 +407,64 @@ def adjoint_leg():
                                 "device_MB": sq["device_bytes"] / 1e6, "same_gradient": bool(np.array_equal(g, gq))}}
 
 
-def precision_report(layers=1000, t_end=60e-6):
+def hex_lengths(x, conn, chunk=262144):
+    """the hex8 Courant length V / A_max of every element (the kernels' and
+    the oracle's build-defined length), vectorised in chunks"""
+    import numpy as np
+    sg = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]],
+                  dtype=float)
+    g = sg / np.sqrt(3.0)
+    dn0 = np.zeros((8, 8, 3))
+    for p in range(8):
+        for k in range(8):
+            f = (1 + sg[k] * g[p]) / 2.0
+            for d in range(3):
+                dn0[p, k, d] = sg[k, d] / 2.0 * np.prod([f[e] for e in range(3) if e != d])
+    faces = np.array([[0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7], [7, 4, 5, 6], [2, 1, 0, 3]])
+    out = np.empty(conn.shape[0])
+    for c0 in range(0, conn.shape[0], chunk):
+        xe = x[conn[c0:c0 + chunk]]  # [n, 8, 3]
+        J = np.einsum("pkd,nki->npid", dn0, xe)
+        V = np.linalg.det(J).sum(axis=1)
+        d1 = xe[:, faces[:, 2]] - xe[:, faces[:, 0]]
+        d2 = xe[:, faces[:, 3]] - xe[:, faces[:, 1]]
+        A = 0.5 * np.linalg.norm(np.cross(d1, d2), axis=2).max(axis=1)
+        out[c0:c0 + chunk] = V / A
+    return out
+
+
+def courant_element(mesh, d, form):
+    """where the Courant time step is set: the element of least V / A_max,
+    its centroid, axial and radial extent against the initial ones, and the
+    largest plastic strain and temperature of its points"""
+    import numpy as np
+    from redynela_amd import _lib as L
+    x, x0, conn = d.coords, mesh["x"], mesh["conn"]
+    ne = conn.shape[0]
+    lens = hex_lengths(x, conn)
+    e = int(np.argmin(lens))
+    xe, xe0 = x[conn[e]], x0[conn[e]]
+    c = xe.mean(axis=0)
+    ext = lambda p: (float(np.ptp(p[:, 2])), float(np.ptp(np.hypot(p[:, 0], p[:, 1]))))
+    (hz, hr), (hz0, hr0) = ext(xe), ext(xe0)
+    # point fields: [element][point] in the FP64 form, [point][element] in the mixed one
+    pt = (lambda f: f.reshape(ne, 8)[e]) if form == "fp64" else (lambda f: f.reshape(8, ne)[:, e])
+    out = {"element": e, "length_m": float(lens[e]), "initial_length_m": float(hex_lengths(x0, conn[e:e + 1])[0]),
+           "centroid_r_m": float(np.hypot(c[0], c[1])), "centroid_z_m": float(c[2]),
+           "axial_extent_m": hz, "initial_axial_extent_m": hz0, "radial_extent_m": hr,
+           "initial_radial_extent_m": hr0, "evp_max_of_points": float(pt(d.field(L.F_EVP)).max())}
+    T = d.field(L.F_TEMP)
+    if T.size:
+        out["temperature_max_of_points_K"] = float(pt(T).max())
+    return out
+
+
+def precision_report(layers=1000, t_end=60e-6, forms=("fp64", "mixed")):
     """The O-grid bar of 4,000 hexahedra per section (40x40 core + 4 rings of
     40x15; 1,000 layers = 4.0M elements), 250 m/s, Johnson-Cook with rate and
     adiabatic heating, to t_end in the FP64 and the mixed form: final length,
-    mushroom radius and peak plastic strain, and their relative differences."""
+    mushroom radius and peak plastic strain, their relative differences, and
+    the element that sets the Courant time step at the end."""
     import numpy as np
     import redynela_amd as R
     mesh = R.mesh.ogrid_bar(40, 15, layers, 3.2e-3, 32.4e-3, 0.4, 250.0)
@@ -414,7 +472,7 @@ def precision_report(layers=1000, t_end=60e-6):
     out = {"workload": "hex8 O-grid 4,000 el/section x %d layers (%d elements), 250 m/s, Johnson-Cook "
                        "A=90MPa B=292MPa n=0.31 C=0.025 m=1.09 adiabatic, to %.0f us"
                        % (layers, mesh["conn"].shape[0], t_end * 1e6)}
-    for form in ("fp64", "mixed"):
+    for form in forms:
         h = R.IsoHardJohnsonCook(90e6, 292e6, 0.31, C=0.025, m=1.09, depsp0=1.0, Tm=1356.0, T0=293.0)
         h.enableRateAndThermal(383.0, 0.9)
         sol = R.ExplicitSolverCH()
@@ -425,6 +483,7 @@ def precision_report(layers=1000, t_end=60e-6):
         d.set_option(2, 0)
         t0 = last = time.perf_counter()
         d.begin(t_end)
+        dt0 = d.time()["dt"]
         while d.step(256):
             if time.perf_counter() - last > 15.0:
                 last = time.perf_counter()
@@ -432,14 +491,17 @@ def precision_report(layers=1000, t_end=60e-6):
                 print("  %s: increment %d, t = %.3e s, dt = %.3e s" % (form, tt["increment"], tt["t"], tt["dt"]),
                       file=sys.stderr, flush=True)
         el = time.perf_counter() - t0
-        n = d.time()["increment"]
+        tm = d.time()
         x = d.coords
-        out[form] = {"increments": n, "solve_s": el, "length": float(x[:, 2].max()),
-                     "radius": float(np.hypot(x[base, 0], x[base, 1]).max()), "max_evp": float(d.evp.max())}
+        out[form] = {"increments": tm["increment"], "t_end": tm["t"], "solve_s": el, "length": float(x[:, 2].max()),
+                     "radius": float(np.hypot(x[base, 0], x[base, 1]).max()), "max_evp": float(d.evp.max()),
+                     "first_dt": dt0, "last_computed_dt": tm["computed_dt"],
+                     "courant_element": courant_element(mesh, d, form)}
         d.close()
-    a, b = out["fp64"], out["mixed"]
-    out["rel_err"] = {k: abs(b[k] - a[k]) / abs(a[k]) for k in ("length", "radius", "max_evp")}
-    out["speedup_mixed_vs_fp64"] = a["solve_s"] / b["solve_s"]
+    if "fp64" in out and "mixed" in out:
+        a, b = out["fp64"], out["mixed"]
+        out["rel_err"] = {k: abs(b[k] - a[k]) / abs(a[k]) for k in ("length", "radius", "max_evp")}
+        out["speedup_mixed_vs_fp64"] = a["solve_s"] / b["solve_s"]
     return out
 
 
