@@ -62,17 +62,20 @@ __device__ __forceinline__ bool stress_update(const ElemArgs &a, const T9 &dEps,
 #pragma unroll
   for (int k = 0; k < 9; k++) S.v[k] += g2 * dev.v[k];
   if (!a.yield) return true;
-  const double Sigv0 = a.A + a.B * det_pow(evp + kSqrt23 * 0.0, a.n);
+  // pow(evp, n) and pow(evp, n - 1) share one logarithm (DetPowBase: the same doubles)
+  DetPowBase b0(evp + kSqrt23 * 0.0);
+  const double Sigv0 = a.A + a.B * b0.pow(a.n);
   const double Strial = kSqrt32 * sqrt(sumsq(S));
   if (!(Strial > Sigv0)) return true;
-  double h = a.n * a.B * det_pow(evp + kSqrt23 * 0.0, a.nm1);
+  double h = a.n * a.B * b0.pow(a.nm1);
   double gamma = (sqrt(sumsq(S)) - kSqrt23 * Sigv0) / (2.0 * a.G * (1.0 + (h) / (3.0 * a.G)));
   int iteration = 0;
   while (true)
   {
     const double ev = evp + kSqrt23 * gamma;
-    const double Sigv = a.A + a.B * det_pow(ev, a.n);
-    h = a.n * a.B * det_pow(ev, a.nm1);
+    DetPowBase bi(ev);
+    const double Sigv = a.A + a.B * bi.pow(a.n);
+    h = a.n * a.B * bi.pow(a.nm1);
     const double dfun = kSqrt32 * 2.0 * a.G + kSqrt23 * h;
     const double fun = dfun * gamma - (Strial - Sigv + h * kSqrt23 * gamma);
     bool run = dabs(fun / Sigv) > kRRPrec;
@@ -127,11 +130,12 @@ __device__ __forceinline__ bool stress_update_jc(const ElemArgs &a, const T9 &dE
 #pragma unroll
   for (int k = 0; k < 9; k++) S.v[k] += g2 * dev.v[k];
   const double theta = jc_theta(a, T);
-  const double sy0 = (a.A + a.B * det_pow(evp, a.n)) * theta;
+  DetPowBase b0(evp);
+  const double sy0 = (a.A + a.B * b0.pow(a.n)) * theta;
   const double nrm = sqrt(sumsq(S));
   const double Strial = kSqrt32 * nrm;
   if (!(Strial > sy0)) return true;
-  const double h0 = a.n * a.B * det_pow(evp, a.nm1) * theta;
+  const double h0 = a.n * a.B * b0.pow(a.nm1) * theta;
   double gamma = (nrm - kSqrt23 * sy0) / (2.0 * a.G * (1.0 + h0 / (3.0 * a.G)));
   double lo = 0.0, hi = Strial / (kSqrt32 * 2.0 * a.G);
   if (!(gamma > lo && gamma < hi)) gamma = 0.5 * (lo + hi);
@@ -140,8 +144,9 @@ __device__ __forceinline__ bool stress_update_jc(const ElemArgs &a, const T9 &dE
   while (true)
   {
     const double ev = evp + kSqrt23 * gamma;
-    const double base = a.A + a.B * det_pow(ev, a.n);
-    const double dbase = a.n * a.B * det_pow(ev, a.nm1);
+    DetPowBase bi(ev);
+    const double base = a.A + a.B * bi.pow(a.n);
+    const double dbase = a.n * a.B * bi.pow(a.nm1);
     double rf = 1.0, drf = 0.0;
     if (a.C != 0.0 && gamma > 0.0)
     {

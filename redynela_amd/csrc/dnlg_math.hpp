@@ -93,6 +93,26 @@ DNL_HD double det_pow(double x, double y)
   return det_exp(y * det_log(x));
 }
 
+// det_pow(x, y) of one base for several exponents with one logarithm: each
+// result is det_pow(x, y) bit for bit (the special cases take det_pow's own
+// branches; otherwise both compute det_exp(y * det_log(x)))
+struct DetPowBase
+{
+  double x, L;
+  bool have;
+  DNL_HD explicit DetPowBase(double x_) : x(x_), L(0.0), have(false) {}
+  DNL_HD double pow(double y)
+  {
+    if (y == 0.0 || y == 1.0 || x == 1.0 || !(x > 0.0)) return det_pow(x, y);
+    if (!have)
+    {
+      L = det_log(x);
+      have = true;
+    }
+    return det_exp(y * L);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // 3x3 tensors, row-major v[3i+j] (LA/Tensor2.h)
 // ---------------------------------------------------------------------------
