@@ -234,9 +234,9 @@ long long dnlg_plot_rows(dnlg_handle *h, double *out, long long max_rows);
  * FE/Structure.C:601-613): at each endStep whose time tm reaches the file's
  * next save time (from `start`, every `every`, up to stop + every when
  * stop > 0) a row (tm, value of each item) is kept.  Items are (node index,
- * field, component) with Node::getNodalValue's fields (FE/Node.C:326-342) as
+ * field, component) with Node::getNodalValue's fields (FE/Node.C:326-418) as
  * DNLG_HF_* codes and its components: 0 the norm of a vector, 1..3 one
- * component; vector fields are read from the Current nodal field as the
+ * component (element fields: below); vector fields are read from the Current nodal field as the
  * reference does at endStep (the previous increment's values), coordinates
  * after the update.  init_temp is the material's initial temperature
  * (Material::setInitTemperature) for the temperature fields.  Rows are read
@@ -255,6 +255,17 @@ long long dnlg_plot_rows(dnlg_handle *h, double *out, long long max_rows);
 #define DNLG_HF_COORDS 10
 #define DNLG_HF_MASS 11
 #define DNLG_HF_TEMPERATURE_INIT 12
+/* element fields (FE/Node.C:350-418, FP64 form): the points' tensors
+ * extrapolated to the node with the element's integrationPointsToNode table
+ * and averaged over the node's elements (FE/Element.C:1180-1198);
+ * component 0 is the von Mises equivalent of the stress or the norm
+ * sqrt(2/3 e:e) of a strain, 10 i + j (i, j in 1..3) the entry (i, j).  The
+ * strain fields need the tracked point strains: on a handle that has not
+ * begun they turn DNLG_OPT_TRACK_STRAIN on. */
+#define DNLG_HF_STRESS 13
+#define DNLG_HF_STRAIN 14
+#define DNLG_HF_STRAIN_INC 15
+#define DNLG_HF_PLASTIC_STRAIN 16
 int dnlg_history_file_add(dnlg_handle *h, double start, double stop, double every, int n_items,
                           const long long *nodes, const int *fields, const int *comps, double init_temp, int *id);
 long long dnlg_history_file_rows(dnlg_handle *h, int id, double *out, long long max_rows);

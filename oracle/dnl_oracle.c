@@ -1659,6 +1659,75 @@ long dnlo_solve(dnlo *m, double upTime)
   return n;
 }
 
+/* ---- Node::getNodalValue of the element fields (FE/Node.C:350-418) ----
+ * get_IntegrationValue_atNode (FE/Element.C:1180-1198): over the node's
+ * elements in the order Domain::initSolve attached them (element order,
+ * FE/Domain.C:1090-1107), VAR += integrationPointsToNode(loc)(pt) * point
+ * value, then VAR / (number of elements); Node::getNodalValue adds that tensor
+ * once per element and divides by the count again.  Tables:
+ * ElHex8n3D (FE/ElHex8n3D.C:33-72), ElQua4nAx (FE/ElQua4nAx.C:38-48).
+ * field: 13 stress (Sig), 14 strain (Eps), 15 strainInc (dEps), 16
+ * plasticStrain (EpsPlas); comp 0 the equivalent (misesEquivalent for the
+ * stress, BT/Math.h:87-88; equivalent() for a strain, LA/Tensor2.h:359-372)
+ * or 10 i + j for entry (i, j). */
+double dnlo_nodal_value(const dnlo *m, long node, int field, int comp)
+{
+#define O_C5P ((5.0 + 3.0 * sqrt(3.0)) / 4.0)
+#define O_C5M ((5.0 - 3.0 * sqrt(3.0)) / 4.0)
+#define O_R3P ((sqrt(3.0) + 1.0) / 4.0)
+#define O_R3M ((sqrt(3.0) - 1.0) / 4.0)
+  const double hex[8][8] = {
+      {O_C5P, -O_R3P, O_R3M, -O_R3P, -O_R3P, O_R3M, O_C5M, O_R3M},
+      {-O_R3P, O_C5P, -O_R3P, O_R3M, O_R3M, -O_R3P, O_R3M, O_C5M},
+      {O_R3M, -O_R3P, O_C5P, -O_R3P, O_C5M, O_R3M, -O_R3P, O_R3M},
+      {-O_R3P, O_R3M, -O_R3P, O_C5P, O_R3M, O_C5M, O_R3M, -O_R3P},
+      {-O_R3P, O_R3M, O_C5M, O_R3M, O_C5P, -O_R3P, O_R3M, -O_R3P},
+      {O_R3M, -O_R3P, O_R3M, O_C5M, -O_R3P, O_C5P, -O_R3P, O_R3M},
+      {O_C5M, O_R3M, -O_R3P, O_R3M, O_R3M, -O_R3P, O_C5P, -O_R3P},
+      {O_R3M, O_C5M, O_R3M, -O_R3P, -O_R3P, O_R3M, -O_R3P, O_C5P}};
+  const double qp = 1.0 + sqrt(3.0) / 2.0, qm = 1.0 - sqrt(3.0) / 2.0, qh = 1.0 / 2.0;
+  const double quad[4][4] = {{qp, -qh, qm, -qh}, {-qh, qp, -qh, qm}, {qm, -qh, qp, -qh}, {-qh, qm, -qh, qp}};
+#undef O_C5P
+#undef O_C5M
+#undef O_R3P
+#undef O_R3M
+  const double *src = field == 13 ? m->Sig : field == 14 ? m->Eps : field == 15 ? m->dEps : m->EpsPlas;
+  double V[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  long nb = 0;
+  for (long e = 0; e < m->ne; e++)
+  {
+    int loc = -1;
+    for (int i = 0; i < m->nen; i++)
+      if (m->conn[e * m->nen + i] == node) loc = i;
+    if (loc < 0) continue;
+    for (int pt = 0; pt < m->nip; pt++)
+    {
+      const double c = m->dim == 3 ? hex[loc][pt] : quad[loc][pt];
+      const double *T = src + 9 * (e * m->nip + pt);
+      for (int k = 0; k < 9; k++) V[k] += c * T[k];
+    }
+    nb++;
+  }
+  if (nb == 0) return 0.0;
+  double V1[9];
+  for (int k = 0; k < 9; k++)
+  {
+    const double v = V[k] / nb;
+    double sum = 0.0;
+    for (long j = 0; j < nb; j++) sum += v;
+    V1[k] = sum / nb;
+  }
+  if (comp == 0)
+  {
+    if (field == 13)
+      return sqrt((O_SQ(V1[0] - V1[4]) + O_SQ(V1[0] - V1[8]) + O_SQ(V1[4] - V1[8]) +
+                   6.0 * (O_SQ(V1[1]) + O_SQ(V1[2]) + O_SQ(V1[5]))) / 2.0);
+    return sqrt(2. * (V1[0] * V1[0] + V1[1] * V1[1] + V1[2] * V1[2] + V1[3] * V1[3] + V1[4] * V1[4] +
+                      V1[5] * V1[5] + V1[6] * V1[6] + V1[7] * V1[7] + V1[8] * V1[8]) / 3.);
+  }
+  return V1[3 * (comp / 10 - 1) + (comp % 10 - 1)];
+}
+
 /* ---- state access ---- */
 enum { F_X = 0, F_V, F_A, F_DU, F_D, F_FINT, F_MASS, F_S, F_EVP, F_P, F_EPS, F_EPSPLAS, F_SIG, F_X0, F_FE, F_TEMP, F_DEPS };
 long dnlo_get(const dnlo *m, int field, double *out)

@@ -23,7 +23,10 @@ TS_COURANT, TS_PULSE = 0, 1
 HF_FIELDS = {"density": (0, False), "densityInc": (1, False), "energy": (2, False), "energyInc": (3, False),
              "speed": (4, True), "speedInc": (5, True), "force": (6, True), "temperature": (7, False),
              "displacementInc": (8, True), "displacement": (9, True), "coords": (10, True), "mass": (11, False),
-             "temperatureInit": (12, False)}
+             "temperatureInit": (12, False),
+             # element fields: components 0 (equivalent) or 11 .. 33
+             "stress": (13, "tensor"), "strain": (14, "tensor"), "strainInc": (15, "tensor"),
+             "plasticStrain": (16, "tensor")}
 
 # every symbol include/dnlg.h declares
 EXPORTS = [

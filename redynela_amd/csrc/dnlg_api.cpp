@@ -163,6 +163,7 @@ static void free_all(dnlg_handle *h)
     if (f.ring) (void)hipFree(f.ring);
   if (h->hf_dev) (void)hipFree(h->hf_dev);
   if (h->hi_dev) (void)hipFree(h->hi_dev);
+  if (h->hel_dev) (void)hipFree(h->hel_dev);
   adj_free(h->adj);
   h->adj = nullptr;
   void *ptrs[] = {h->x, h->v, h->a, h->du, h->d, h->mass, h->fint, h->fe, h->S, h->evp, h->p, h->eps, h->epsplas, h->deps,
@@ -539,6 +540,36 @@ static HistArgs hist_args(dnlg_handle *h)
   a.st = h->st;
   a.density = h->mat.density;
   a.T0 = h->init_temp;
+  a.hel = h->hel_dev;
+  a.S = h->S;
+  a.p = h->p;
+  a.eps = h->eps;
+  a.epsplas = h->epsplas;
+  a.deps = h->deps;
+  a.nipt = h->ne * h->nip;
+  a.nip = h->nip;
+  a.nen = h->nen;
+  // integrationPointsToNode of ElHex8n3D (FE/ElHex8n3D.C:33-72) and ElQua4nAx
+  // (FE/ElQua4nAx.C:38-48), each coefficient the expression the reference's
+  // tables evaluate
+  if (h->dim == 3)
+  {
+    const double p5 = (5.0 + 3.0 * sqrt(3.0)) / 4.0, m5 = (5.0 - 3.0 * sqrt(3.0)) / 4.0;
+    const double r1 = (sqrt(3.0) + 1.0) / 4.0, q1 = (sqrt(3.0) - 1.0) / 4.0;
+    const double t[8][8] = {{p5, -r1, q1, -r1, -r1, q1, m5, q1}, {-r1, p5, -r1, q1, q1, -r1, q1, m5},
+                            {q1, -r1, p5, -r1, m5, q1, -r1, q1}, {-r1, q1, -r1, p5, q1, m5, q1, -r1},
+                            {-r1, q1, m5, q1, p5, -r1, q1, -r1}, {q1, -r1, q1, m5, -r1, p5, -r1, q1},
+                            {m5, q1, -r1, q1, q1, -r1, p5, -r1}, {q1, m5, q1, -r1, -r1, q1, -r1, p5}};
+    for (int i = 0; i < 8; i++)
+      for (int j = 0; j < 8; j++) a.coef[i][j] = t[i][j];
+  }
+  else
+  {
+    const double p1 = 1.0 + sqrt(3.0) / 2.0, m1 = 1.0 - sqrt(3.0) / 2.0, hf = 1.0 / 2.0;
+    const double t[4][4] = {{p1, -hf, m1, -hf}, {-hf, p1, -hf, m1}, {m1, -hf, p1, -hf}, {-hf, m1, -hf, p1}};
+    for (int i = 0; i < 8; i++)
+      for (int j = 0; j < 8; j++) a.coef[i][j] = (i < 4 && j < 4) ? t[i][j] : 0.0;
+  }
   return a;
 }
 
@@ -899,6 +930,33 @@ extern "C" int dnlg_history_file_add(dnlg_handle *h, double start, double stop, 
       h->err = "HistoryFile::append: bad node, field or component";
       return DNLG_E_ARG;
     }
+    if (fields[k] >= HF_STRESS)
+    {
+      // element fields: component 0 (equivalent) or 10 i + j with i, j in 1..3 (FE/Node.C:362-365)
+      const int c = comps[k], i = c / 10, j = c % 10;
+      if (c != 0 && !(i >= 1 && i <= 3 && j >= 1 && j <= 3))
+      {
+        h->err = "Node::getNodalValue: No sense for component of a tensor (0, or 11 .. 33)";
+        return DNLG_E_ARG;
+      }
+      if (h->form != DNLG_FORM_FP64)
+      {
+        h->err = "dnlg_history_file_add: element fields exist in the FP64 form only";
+        return DNLG_E_ARG;
+      }
+      if (fields[k] != HF_STRESS && !h->track_strain)
+      {
+        // strain fields need the tracked point strains from the first pass on
+        if (h->begun || h->inc != 0 || h->mass_done)
+        {
+          h->err = "dnlg_history_file_add: strain fields need DNLG_OPT_TRACK_STRAIN set before the first step";
+          return DNLG_E_STATE;
+        }
+        const int rc = dnlg_set_option(h, DNLG_OPT_TRACK_STRAIN, 1);
+        if (rc) return rc;
+      }
+      continue;
+    }
     const bool vec = fields[k] == HF_SPEED || fields[k] == HF_SPEED_INC || fields[k] == HF_FORCE ||
                      fields[k] == HF_DISPLACEMENT_INC || fields[k] == HF_DISPLACEMENT || fields[k] == HF_COORDS;
     if (vec && comps[k] > 3)
@@ -928,15 +986,34 @@ extern "C" int dnlg_history_file_add(dnlg_handle *h, double start, double stop, 
   f.count = n_items;
   f.cap = (int)PLOT_CAP;
   DNLG_HIP(h, dalloc(&f.ring, (size_t)PLOT_CAP * (1 + n_items)));
-  for (int k = 0; k < n_items; k++) h->hitems.push_back(HistItem{(int)nodes[k], fields[k], comps[k], 0});
+  for (int k = 0; k < n_items; k++)
+  {
+    HistItem it{(int)nodes[k], fields[k], comps[k], 0, 0, 0};
+    if (fields[k] >= HF_STRESS)
+    {
+      // the node's elements in element order (Node::addElement in Domain::initSolve,
+      // FE/Domain.C:1090-1107) with the node's local number in each
+      it.el_off = (int)h->hel.size();
+      for (long long e = 0; e < h->ne; e++)
+        for (int L = 0; L < h->nen; L++)
+          if ((long long)(h->fpos_host[e * h->nen + L] % h->nn) == nodes[k]) h->hel.push_back(int2{(int)e, L});
+      it.el_cnt = (int)h->hel.size() - it.el_off;
+    }
+    h->hitems.push_back(it);
+  }
   h->hf.push_back(f);
   h->hf_drained.push_back(0);
   h->hf_rows.emplace_back();
   h->init_temp = init_temp;
   if (h->hf_dev) (void)hipFree(h->hf_dev);
   if (h->hi_dev) (void)hipFree(h->hi_dev);
+  if (h->hel_dev) (void)hipFree(h->hel_dev);
   h->hf_dev = nullptr;
   h->hi_dev = nullptr;
+  h->hel_dev = nullptr;
+  DNLG_HIP(h, dalloc(&h->hel_dev, h->hel.size()));
+  if (!h->hel.empty())
+    DNLG_HIP(h, hipMemcpy(h->hel_dev, h->hel.data(), sizeof(int2) * h->hel.size(), hipMemcpyHostToDevice));
   DNLG_HIP(h, dalloc(&h->hf_dev, h->hf.size()));
   DNLG_HIP(h, dalloc(&h->hi_dev, h->hitems.size()));
   DNLG_HIP(h, hipMemcpy(h->hf_dev, h->hf.data(), sizeof(HistFile) * h->hf.size(), hipMemcpyHostToDevice));

@@ -66,6 +66,8 @@ struct dnlg_handle
   std::vector<HistItem> hitems;
   HistFile *hf_dev = nullptr;
   HistItem *hi_dev = nullptr;
+  std::vector<int2> hel;          // element fields: (element, local node) pairs of the items
+  int2 *hel_dev = nullptr;
   std::vector<long long> hf_drained;
   std::vector<std::vector<double>> hf_rows;
   double init_temp = 0.0;

@@ -24,7 +24,8 @@ setSaveTimes, addSolver), NodeSet and ElementSet (add(i), add(i, j),
 add(i, j, step): FE/Select.C:151-212), Select, ElQua4nAx, ElHex8n3D,
 Material, IsoHardElastoplastic, IsoHardJohnsonCook, BoundaryRestrain,
 BoundarySpeed, ExplicitSolverCH (Courant or Pulse time step), HistoryFile
-(nodal fields; the element fields stress/strain/... are refused) and
+(nodal fields and the element fields stress, strain, strainInc,
+plasticStrain) and
 Structure::addHistoryFile.
 
 `load(path)` returns a Deck (mesh arrays, material, BCs, solver and save
@@ -428,11 +429,13 @@ class HistoryFileObj(Obj):
         if not isinstance(ns, NodeSet):
             raise DeckError("HistoryFile::append: not a NodeSet")
         if field not in HF_FIELDS:
-            raise DeckError("HistoryFile::append: field %s is not supported by this reader "
-                            "(element fields need the nodal extrapolation)" % field)
-        vec = HF_FIELDS[field][1]
+            raise DeckError("getNodalValue: undefined field %s" % field)
+        kind = HF_FIELDS[field][1]
         comp = _idx(comp)
-        if comp < 0 or (vec and comp > 3) or (not vec and comp > 1):
+        if kind == "tensor":
+            if comp != 0 and not (1 <= comp // 10 <= 3 and 1 <= comp % 10 <= 3):
+                raise DeckError("Node::getNodalValue: No sense for component %d of %s" % (comp, field))
+        elif comp < 0 or (kind and comp > 3) or (not kind and comp > 1):
             raise DeckError("Node::getNodalValue::get: No sense for component %d of %s" % (comp, field))
         self.items.append((list(ns.items), field, comp))
 

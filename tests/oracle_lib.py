@@ -60,6 +60,8 @@ def lib():
         L.dnlo_set_threads.argtypes = [C.c_int]
         L.dnlo_set_strategy.restype = C.c_int
         L.dnlo_set_strategy.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.dnlo_nodal_value.restype = C.c_double
+        L.dnlo_nodal_value.argtypes = [C.c_void_p, C.c_long, C.c_int, C.c_int]
         L.dnlo_jobs.restype = C.c_long
         L.dnlo_jobs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.dnlo_init.restype = C.c_int
@@ -221,6 +223,10 @@ class Model:
 
     def compute_internal_forces(self):
         return lib().dnlo_compute_internal_forces(self.h)
+
+    def nodal_value(self, node, field, comp):
+        """Node::getNodalValue of an element field: 13 stress, 14 strain, 15 strainInc, 16 plasticStrain"""
+        return lib().dnlo_nodal_value(self.h, int(node), int(field), int(comp))
 
     def set_strategy(self, strategy, threads):
         """Domain::computeInternalForces strategy: 0 (element order) or the reference's case 2, 3, 6, 7"""

@@ -183,8 +183,15 @@ def test_history_file_items_and_names():
     for nums, field, comp in hf.items:
         h.append([d._rank[n] for n in nums], field, comp, numbers=nums)
     assert h.columns == ["speed_3_n%d" % n for n in range(28, 37)] + ["displacement", "coords_3", "density", "mass"]
-    with pytest.raises(deck.DeckError, match="stress is not supported"):
-        deck.load('Structure s; s.createNode(1,0,0,0); NodeSet n; n.add(1); HistoryFile h; h.append(n, "stress");',
+    # element fields take component 0 (equivalent) or 11 .. 33 (an entry)
+    d = deck.load('Structure s; s.createNode(1,0,0,0); NodeSet n; n.add(1); HistoryFile h; h.append(n, "stress");'
+                  'h.append(n, "plasticStrain", 23); s.addHistoryFile(h);', text=True)
+    assert [it[1:] for it in d.history_files[0].items] == [("stress", 0), ("plasticStrain", 23)]
+    with pytest.raises(deck.DeckError, match="No sense for component 14 of stress"):
+        deck.load('Structure s; s.createNode(1,0,0,0); NodeSet n; n.add(1); HistoryFile h; h.append(n, "stress", 14);',
+                  text=True)
+    with pytest.raises(deck.DeckError, match="undefined field flux"):
+        deck.load('Structure s; s.createNode(1,0,0,0); NodeSet n; n.add(1); HistoryFile h; h.append(n, "flux");',
                   text=True)
     with pytest.raises(deck.DeckError, match="No sense for component 4"):
         deck.load('Structure s; s.createNode(1,0,0,0); NodeSet n; n.add(1); HistoryFile h; h.append(n, "speed", 4);',
