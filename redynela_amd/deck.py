@@ -752,6 +752,8 @@ class Deck:
                 raise DeckError("restart needs the result files' base name (results=)")
             t0 = RF.restore(dom, "%s.%03d.bin" % (results, int(restart)))
             writer.count = int(restart) + 1
+            # the restored Current field's ro and T (0 when the file was written at odd parity)
+            writer.density, writer.T0 = dom.restored_density_T
             resume = (t0, (int(restart) + 1) * save[2])
 
         def solve(up_time):

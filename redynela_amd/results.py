@@ -58,7 +58,9 @@ def _sep():
 
 def node_block(coords, v, a, du, d, increment, density, T0=0.0, fe=None):
     """[nn][23] Node::write records; the Current field's ro and T follow the
-    swap parity of `increment` (the number of completed increments)."""
+    swap parity of `increment` (the number of completed increments): the
+    `density` and `T0` given (scalars or per node: after a restart, the
+    restored file's values) after an even number, 0 after an odd one."""
     nn = coords.shape[0]
     rec = np.zeros((nn, NODE_DOUBLES))
     rec[:, 0:3] = coords
@@ -146,6 +148,10 @@ def restore(dom, path):
         dom.set_field(L.F_EPS, ips[:, 20:29].T)
         dom.set_field(L.F_EPSPLAS, ips[:, 38:47].T)
     dom.set_time(rec["time"])
+    # Node::read puts the file's ro and T into the Current field (FE/Node.C:148);
+    # the other field stays cleared, so from here on the files carry these
+    # values after an even number of increments and 0 after an odd number
+    dom.restored_density_T = (nodes[:, 3].copy(), nodes[:, 7].copy())
     return rec["time"]
 
 
@@ -175,7 +181,7 @@ class ResultFiles:
     """Incremental result files `<base>.NNN.bin` (FE/io_Structure.C:60-68)."""
 
     def __init__(self, base, density, T0=0.0):
-        self.base, self.density, self.T0 = base, float(density), float(T0)
+        self.base, self.density, self.T0 = base, density, T0
         self.count = 1
         self.paths = []
 

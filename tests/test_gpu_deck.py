@@ -240,34 +240,55 @@ def test_deck_mixed_form_runs():
     b.close()
 
 
-def test_deck_restart_matches_oracle_restart(tmp_path):
+@pytest.mark.parametrize("parity", ["even", "odd"])
+def test_deck_restart_matches_oracle_restart(tmp_path, parity):
     """DynELA_solve's restart (Sources/Applications/DynELA_solve/main.C:106-130):
-    the axisymmetric deck is run, then restarted from its 1st result file; the
-    files it writes from there on equal, byte for byte, those of the oracle
-    restarted from the same file (state read back, mass from the restored
-    coordinates, fresh time-step history), and they differ from the files of
-    the uninterrupted run, as the reference's restart is no exact continuation"""
+    the axisymmetric deck is run, then restarted from its result file nfile;
+    the files it writes from there on equal, byte for byte, those of the
+    oracle restarted from the same file (state read back, mass from the
+    restored coordinates, fresh time-step history), and they differ from the
+    files of the uninterrupted run, as the reference's restart is no exact
+    continuation.  The nodal ro and T columns follow Node::read (FE/Node.C:148):
+    the file's values go into the Current field and the other field stays
+    cleared, so after k increments from the restart they are the file's when
+    k is even and 0 when it is odd -- stated here, not taken from the writer.
+    The restart file is the first one written after an even (ro = density)
+    or an odd (ro = 0) number of increments."""
     from redynela_amd import deck
     from redynela_amd import results as RF
-    say("case: axi deck restarted from result file 1, GPU against the oracle")
     d = deck.load(os.path.join(HERE, "decks", "taylor_axi_small.dyn"))
     _, mat, sol = d.model()
-    base = str(tmp_path / "run" / "axi")
-    d.run(results=base).close()
-    full = sorted(f for f in os.listdir(tmp_path / "run") if f.endswith(".bin"))
-    assert len(full) >= 3
-    # GPU restart from file 1 into a new base: files 2.. are written
+    o0 = O.from_deck(d)
+    nn0, nipt0 = o0.nn, o0.get(O.F_EVP).size
+    want_ro = (lambda r: r != 0.0) if parity == "even" else (lambda r: r == 0.0)
+    # the deck's save period, or another one whose save points fall at the parity wanted
+    nfile = None
+    for nsaves in (3, 4, 5, 7, 9):
+        d.save = (0.0, sol.stop, sol.stop / nsaves)
+        base = str(tmp_path / ("run%d" % nsaves) / "axi")
+        d.run(results=base).close()
+        full = sorted(f for f in os.listdir(os.path.dirname(base)) if f.endswith(".bin"))
+        cands = [k for k in range(1, len(full))
+                 if want_ro(RF.read("%s.%03d.bin" % (base, k), nn0, nipt0)["nodes"][0, 3])]
+        if cands:
+            nfile = cands[0]
+            break
+    assert nfile is not None, "no result file of %s parity before the last" % parity
+    run_dir = os.path.dirname(base)
+    say("case: axi deck restarted from result file %d (%s parity), GPU against the oracle" % (nfile, parity))
+    # GPU restart from file nfile into a new base: files nfile + 1 .. are written
     rbase = str(tmp_path / "restart" / "axi")
     os.makedirs(tmp_path / "restart")
     import shutil
-    shutil.copy("%s.001.bin" % base, "%s.001.bin" % rbase)
-    dom = d.run(results=rbase, restart=1)
+    shutil.copy("%s.%03d.bin" % (base, nfile), "%s.%03d.bin" % (rbase, nfile))
+    dom = d.run(results=rbase, restart=nfile)
     dom.close()
     # the oracle, restarted from the same file
     o = O.from_deck(d)
     nn, nipt = o.nn, o.get(O.F_EVP).size
-    rec = RF.read("%s.001.bin" % base, nn, nipt)
+    rec = RF.read("%s.%03d.bin" % (base, nfile), nn, nipt)
     nodes, ips = rec["nodes"], rec["ips"]
+    ro_file, T_file = nodes[:, 3].copy(), nodes[:, 7].copy()
     for f, c in ((O.F_X, 0), (O.F_V, 8), (O.F_A, 11), (O.F_DU, 17), (O.F_D, 20)):
         o.set(f, nodes[:, c:c + 3])
     o.set(O.F_P, ips[:, 0])
@@ -279,8 +300,11 @@ def test_deck_restart_matches_oracle_restart(tmp_path):
     paths = []
 
     def save(t):
-        _, nb, ib = O.result_blocks(o, mat.density, mat.init_temp)
-        paths.append(str(tmp_path / ("oracle.%03d.bin" % (2 + len(paths)))))
+        _, nb, ib = O.result_blocks(o, 0.0, 0.0)
+        even = o.time()["increment"] % 2 == 0
+        nb[:, 3] = ro_file if even else 0.0
+        nb[:, 7] = T_file if even else 0.0
+        paths.append(str(tmp_path / ("oracle.%03d.bin" % (nfile + 1 + len(paths)))))
         RF.write(paths[-1], t, nb, ib)
 
     def solve(t):
@@ -288,10 +312,12 @@ def test_deck_restart_matches_oracle_restart(tmp_path):
         return o.time()["t"]
 
     save_times = d.save if d.save is not None else (0.0, sol.stop, sol.stop)
-    deck.structure_solve(sol.stop, save_times, solve, save, resume=(rec["time"], 2 * save_times[2]))
+    deck.structure_solve(sol.stop, save_times, solve, save, resume=(rec["time"], (nfile + 1) * save_times[2]))
     gpu = sorted(f for f in os.listdir(tmp_path / "restart") if f.endswith(".bin"))[1:]
-    assert len(gpu) == len(paths) == len(full) - 1
+    assert len(gpu) == len(paths) == len(full) - nfile
     for g, r in zip(gpu, paths):
         assert open(tmp_path / "restart" / g, "rb").read() == open(r, "rb").read(), g
     # not the uninterrupted run's files (fresh mass and time-step history)
-    assert open(tmp_path / "restart" / gpu[-1], "rb").read() != open(tmp_path / "run" / full[-1], "rb").read()
+    assert open(tmp_path / "restart" / gpu[-1], "rb").read() != open(os.path.join(run_dir, full[-1]), "rb").read()
+    say("  restart file %d of %d (save period stop/%d): ro column %s" % (nfile, len(full), nsaves,
+        "density" if ro_file[0] != 0.0 else "0 (odd parity)"))
