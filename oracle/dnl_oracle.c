@@ -58,7 +58,13 @@
  *     matrix equals the published explicit generalized-alpha recurrence
  *     (test_oracle_pins.py); central-difference identity at rho_b = 1 and the
  *     LINEARdbl time-step extrapolation (test_oracle_run.py);
- *   - Pulse power iteration against numpy eigenvalues (test_oracle_pulse.py).
+ *   - Pulse power iteration against numpy eigenvalues (test_oracle_pulse.py);
+ *   - the OpenMP assembly strategies 2, 3, 6, 7: case 6's Fint reassembled in
+ *     numpy from the per-element forces and the partition the oracle reports,
+ *     bit for bit; the partition against numpy's centroid order
+ *     (test_oracle_strategies.py);
+ *   - HistoryFile points-to-nodes extrapolation: exact on trilinear (hex8) and
+ *     bilinear (quad4) fields, node averages (test_oracle_history.py).
  *   Parity unpinned: the Newton iterate sequence for n < 1 (only its
  *   converged consistency is checked), the BoundarySpeed rule, and glibc's
  *   last bits of log / pow (within 1 ulp, test_oracle_math.py).
