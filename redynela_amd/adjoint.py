@@ -1,6 +1,8 @@
 """Adjoint of the axisymmetric explicit solve: gradients of a function of the
 final nodal coordinates with respect to the hardening parameters (A, B, n) of
-sigma_y = A + B evp^n, by the hand-written reverse sweep of libdnlgpu
+sigma_y = A + B evp^n -- or of the Johnson-Cook law with its temperature
+factor and adiabatic heating but no rate term, sigma_y = (A + B evp^n)
+(1 - T*^m) -- by the hand-written reverse sweep of libdnlgpu
 (dnlg_adjoint_forward / dnlg_adjoint_backward) with checkpoint-recompute.
 
 FinalShape is the torch.autograd.Function over it: final (r, z) of every
@@ -12,13 +14,17 @@ import numpy as np
 
 from . import _lib
 from ._lib import DnlgError
-from .domain import Domain, ExplicitSolverCH, IsoHardElastoplastic, Material
+from .domain import Domain, ExplicitSolverCH, IsoHardElastoplastic, IsoHardJohnsonCook, Material
 from .mesh import taylor_axi
 
 # the parameter-identification workload: copper, Johnson-Cook hardening
 # without rate or temperature terms, 20 x 100 quads, 200 m/s, 4,000 increments
 TARGET_THETA = np.array([90e6, 292e6, 0.31])
 ADJ_STEPS, ADJ_CKPT = 4000, 100
+# the same law with the temperature factor and adiabatic heating of the
+# precision workload (m = 1.09, T0 = 293 K, Tm = 1356 K, cp = 383 J/kg/K,
+# Taylor-Quinney 0.9), still without the rate term
+THERMAL = dict(m=1.09, T0=293.0, Tm=1356.0, cp=383.0, tq=0.9)
 
 
 def bar_20x100():
@@ -34,11 +40,14 @@ def perturbed_start(seed=20261203):
 class HardeningAdjoint:
     """One mesh and solver setting; forward(theta) runs nsteps increments from
     the initial state, backward(xbar) returns dL/dtheta for the adjoint xbar
-    of the final coordinates."""
+    of the final coordinates.  thermal=dict(m, T0, Tm, cp, tq) selects the
+    Johnson-Cook temperature factor with adiabatic heating (no rate term)."""
 
-    def __init__(self, mesh, E=115e9, nu=0.3, rho=8930.0, nsteps=ADJ_STEPS, ckpt_every=ADJ_CKPT, rho_b=1.0):
+    def __init__(self, mesh, E=115e9, nu=0.3, rho=8930.0, nsteps=ADJ_STEPS, ckpt_every=ADJ_CKPT, rho_b=1.0,
+                 thermal=None):
         self.mesh, self.E, self.nu, self.rho = mesh, E, nu, rho
         self.nsteps, self.ckpt_every, self.rho_b = int(nsteps), int(ckpt_every), rho_b
+        self.thermal = dict(thermal) if thermal else None
         self.dom = None
         self.stats = None
 
@@ -49,7 +58,13 @@ class HardeningAdjoint:
         s = ExplicitSolverCH()
         s.setTimeStepMethod("Courant")
         s.setDissipation(self.rho_b)
-        self.dom = Domain(self.mesh, Material(self.E, self.nu, self.rho, IsoHardElastoplastic(A, B, n)), s)
+        if self.thermal:
+            t = self.thermal
+            hard = IsoHardJohnsonCook(A, B, n, C=0.0, m=t["m"], Tm=t["Tm"], T0=t["T0"])
+            hard.enableRateAndThermal(t["cp"], t["tq"])
+        else:
+            hard = IsoHardElastoplastic(A, B, n)
+        self.dom = Domain(self.mesh, Material(self.E, self.nu, self.rho, hard), s)
         self.dom.set_option(_lib.OPT_STORE_FINT, 0)
         _lib.check(_lib.lib().dnlg_adjoint_forward(self.dom.h, self.nsteps, self.ckpt_every), self.dom.h)
         return self.dom.coords

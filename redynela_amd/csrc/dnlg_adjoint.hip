@@ -15,8 +15,13 @@
 // decomposition in closed form (rotation angle, eigen-decomposition of U with
 // the Daleckii-Krein divided differences for ln U) and the converged
 // consistency condition of the return, sqrt(3/2) 2G gamma + sigma_y(evp +
-// sqrt(2/3) gamma) = sqrt(3/2)|S_trial|, differentiated implicitly.  One
-// backward increment is four launches: nodal, element, node gather, scalar.
+// sqrt(2/3) gamma) = sqrt(3/2)|S_trial|, differentiated implicitly.  With the
+// Johnson-Cook temperature factor (law 2 without its rate term) sigma_y
+// carries theta(T) = 1 - ((T - T0)/(Tm - T0))^m of the point's temperature
+// before the increment, and the point heats by chi/(rho cp) sigma_y d(evp)
+// after the return: the temperature has an adjoint of its own, and the
+// heating adds its dependence on gamma, evp and (A, B, n).  One backward
+// increment is four launches: nodal, element, node gather, scalar.
 #include "dnlg_elem_f64.hpp"
 
 namespace dnl {
@@ -67,6 +72,23 @@ __global__ __launch_bounds__(256) void k_adj_nodal(AdjArgs g)
     __syncthreads();
   }
   if (threadIdx.x == 0) g.dt_part[blockIdx.x] = red[0];
+}
+
+// theta(T) of the Johnson-Cook law (FE/IsoHardJohnsonCook.h:51-68) as the
+// forward evaluates it, and its derivative
+__device__ __forceinline__ void adj_theta(const ElemArgs &a, double T, double &tf, double &dtf)
+{
+  tf = 1.0;
+  dtf = 0.0;
+  if (a.chi_rho_cp == 0.0 || T < a.T0) return;
+  if (T >= a.Tm)
+  {
+    tf = 0.0;
+    return;
+  }
+  const double ts = (T - a.T0) / (a.Tm - a.T0);
+  tf = 1.0 - pow(ts, a.m);
+  if (ts > 0.0) dtf = -a.m * pow(ts, a.m - 1.0) / (a.Tm - a.T0);
 }
 
 __device__ __forceinline__ double sum4(double v)
@@ -166,17 +188,19 @@ __global__ __launch_bounds__(64) void k_adj_elem(AdjArgs g)
   for (int c = 0; c < 9; c++) s2sum += St[c] * St[c];
   const double snrm = sqrt(s2sum), Strial = kSqrt32 * snrm;
   const double evp = g.evp[ip];
-  const bool plastic = Strial > g.A + g.B * pow(evp, g.n);
+  double tf = 1.0, dtf = 0.0;
+  if (g.jc) adj_theta(ea, g.T[ip], tf, dtf);
+  const bool plastic = Strial > (g.A + g.B * pow(evp, g.n)) * tf;
   double gam = 0.0;
   if (plastic)
   {
-    double h = g.n * g.B * pow(evp, g.n - 1.0);
-    gam = (snrm - kSqrt23 * (g.A + g.B * pow(evp, g.n))) / (g2 * (1.0 + h / (3.0 * g.G)));
+    double h = g.n * g.B * pow(evp, g.n - 1.0) * tf;
+    gam = (snrm - kSqrt23 * (g.A + g.B * pow(evp, g.n)) * tf) / (g2 * (1.0 + h / (3.0 * g.G)));
     for (int it = 0; it < 100; it++)
     {
       const double ev = evp + kSqrt23 * gam;
-      const double sy = g.A + g.B * pow(ev, g.n);
-      h = g.n * g.B * pow(ev, g.n - 1.0);
+      const double sy = (g.A + g.B * pow(ev, g.n)) * tf;
+      h = g.n * g.B * pow(ev, g.n - 1.0) * tf;
       const double phi = kSqrt32 * g2 * gam + sy - Strial;
       if (fabs(phi) <= 1e-15 * sy) break;
       double dg = -phi / (kSqrt32 * g2 + kSqrt23 * h);
@@ -279,19 +303,34 @@ __global__ __launch_bounds__(64) void k_adj_elem(AdjArgs g)
     cb += Spb[c / 3][c % 3] * St[c];
   }
   const double evbo = g.evb[ip];
-  double evbi = evbo, thA = 0.0, thB = 0.0, thN = 0.0;
+  const double Tbo = g.jc ? g.Tb[ip] : 0.0;
+  double evbi = evbo, Tbi = Tbo, thA = 0.0, thB = 0.0, thN = 0.0;
   if (plastic)
   {
-    const double gb = -g2 / snrm * cb + kSqrt23 * evbo;
+    double gb = -g2 / snrm * cb + kSqrt23 * evbo;
     double sb = g2 * gam / (snrm * snrm) * cb;
     const double ev = evp + kSqrt23 * gam;
-    const double evn = pow(ev, g.n);
+    const double evn = pow(ev, g.n), lev = log(ev);
+    const double base = g.A + g.B * evn;
     const double hp = g.n * g.B * pow(ev, g.n - 1.0);
-    const double w = gb / (kSqrt32 * g2 + kSqrt23 * hp);
-    evbi -= w * hp;
-    thA = -w;
-    thB = -w * evn;
-    thN = -w * g.B * evn * log(ev);
+    if (g.jc)
+    {
+      // T' = T + chi/(rho cp) base(ev) theta(T) dep, dep = sqrt(2/3) gamma
+      const double kq = Tbo * ea.chi_rho_cp, dep = kSqrt23 * gam;
+      gb += kq * tf * kSqrt23 * (hp * dep + base);
+      evbi += kq * tf * hp * dep;
+      Tbi += kq * base * dtf * dep;
+      thA += kq * tf * dep;
+      thB += kq * tf * dep * evn;
+      thN += kq * tf * dep * g.B * evn * lev;
+    }
+    // consistency sqrt(3/2) 2G gamma + base(evp + sqrt(2/3) gamma) theta(T) = Strial
+    const double w = gb / (kSqrt32 * g2 + kSqrt23 * hp * tf);
+    evbi -= w * hp * tf;
+    Tbi -= w * base * dtf;
+    thA -= w * tf;
+    thB -= w * tf * evn;
+    thN -= w * tf * g.B * evn * lev;
     sb += kSqrt32 * w;
 #pragma unroll
     for (int c = 0; c < 9; c++) Stb[c] += sb / snrm * St[c];
@@ -376,6 +415,7 @@ __global__ __launch_bounds__(64) void k_adj_elem(AdjArgs g)
 #pragma unroll
     for (int c = 1; c < 9; c++) g.Sb[c * nipt + ip] = Stb[c];
     g.evb[ip] = evbi;
+    if (g.jc) g.Tb[ip] = Tbi;
   }
   // per-node sums over the element's four points; lane pt writes node pt
   double ox0 = 0, ox1 = 0, ou0 = 0, ou1 = 0;
@@ -552,12 +592,13 @@ __global__ __launch_bounds__(256) void k_adj_scalar(AdjArgs g)
     c.dtb_carry = dtb;
 }
 
-// state snapshot / restore: up to 9 arrays copied in one launch (blockIdx.y = array)
+// state snapshot / restore: up to kCopyMax arrays copied in one launch (blockIdx.y = array)
+constexpr int kCopyMax = 12;
 struct CopyList
 {
-  const double *src[9];
-  double *dst[9];
-  long long n[9];
+  const double *src[kCopyMax];
+  double *dst[kCopyMax];
+  long long n[kCopyMax];
 };
 
 __global__ __launch_bounds__(256) void k_copy_list(CopyList L)
@@ -569,8 +610,9 @@ __global__ __launch_bounds__(256) void k_copy_list(CopyList L)
   for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) dst[i] = src[i];
 }
 
-void launch_copy_list(const double *const *src, double *const *dst, const long long *n, int count, hipStream_t s)
+bool launch_copy_list(const double *const *src, double *const *dst, const long long *n, int count, hipStream_t s)
 {
+  if (count < 1 || count > kCopyMax) return false;
   CopyList L{};
   long long mx = 0;
   for (int k = 0; k < count; k++)
@@ -584,6 +626,7 @@ void launch_copy_list(const double *const *src, double *const *dst, const long l
   if (blocks > 1024) blocks = 1024;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_copy_list, dim3((unsigned)blocks, count), dim3(256), 0, s, L);
+  return true;
 }
 
 void launch_adjoint_step(const AdjArgs &g, hipStream_t s)

@@ -14,12 +14,13 @@
 
 namespace dnl {
 void launch_adjoint_step(const AdjArgs &g, hipStream_t s);
-void launch_copy_list(const double *const *src, double *const *dst, const long long *n, int count, hipStream_t s);
+bool launch_copy_list(const double *const *src, double *const *dst, const long long *n, int count, hipStream_t s);
 
-// one state snapshot: x, v, a, du, d ([nn][3] each), S [9][nip ne], evp [nip ne], p [ne], StepState
+// one state snapshot: x, v, a, du, d ([nn][3] each), S [9][nip ne], evp [nip ne], p [ne], StepState,
+// and with the Johnson-Cook temperature factor the point temperature T [nip ne]
 struct Snap
 {
-  long long x, v, a, du, d, S, evp, p, st, size;  // offsets and total size, in doubles
+  long long x, v, a, du, d, S, evp, p, st, T, size;  // offsets and total size, in doubles (T < 0: none)
 };
 
 struct AdjStore
@@ -31,6 +32,7 @@ struct AdjStore
   double *tape = nullptr;   // every + 1 snapshots of the segment being reversed
   double *fin = nullptr;    // the state after the last increment
   double *xb = nullptr, *ub = nullptr, *vb = nullptr, *ab = nullptr, *Sb = nullptr, *evb = nullptr, *pb = nullptr;
+  double *Tb = nullptr;  // adjoint of the point temperature (Johnson-Cook temperature factor)
   double *fb = nullptr, *slot_x = nullptr, *slot_u = nullptr, *dt_part = nullptr, *th_part = nullptr;
   AdjScalars *sc = nullptr;
   float fwd_ms = 0.f;
@@ -40,8 +42,8 @@ struct AdjStore
 void adj_free(AdjStore *s)
 {
   if (!s) return;
-  void *ptrs[] = {s->ckpt, s->tape, s->fin, s->xb, s->ub, s->vb, s->ab, s->Sb, s->evb, s->pb, s->fb, s->slot_x,
-                  s->slot_u, s->dt_part, s->th_part, s->sc};
+  void *ptrs[] = {s->ckpt, s->tape, s->fin, s->xb, s->ub, s->vb, s->ab, s->Sb, s->evb, s->pb, s->Tb, s->fb,
+                  s->slot_x, s->slot_u, s->dt_part, s->th_part, s->sc};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete s;
@@ -50,21 +52,26 @@ void adj_free(AdjStore *s)
 
 static_assert(sizeof(StepState) % 8 == 0, "StepState is copied as doubles");
 
-// the arrays of one snapshot, in doubles: x, v, a, du, d, S, evp, p, StepState
+// the arrays of one snapshot, in doubles: x, v, a, du, d, S, evp, p, StepState (, T)
 static int snap_copy(dnlg_handle *h, const Snap &L, double *snap, bool to_snap)
 {
   const long long n3 = h->nn * 3, nipt = h->ne * h->nip;
-  double *state[9] = {h->x, h->v, h->a, h->du, h->d, h->S, h->evp, h->p, (double *)h->st};
-  const long long off[9] = {L.x, L.v, L.a, L.du, L.d, L.S, L.evp, L.p, L.st};
-  const long long cnt[9] = {n3, n3, n3, n3, n3, 9 * nipt, nipt, h->ne, (long long)(sizeof(StepState) / 8)};
-  const double *src[9];
-  double *dst[9];
-  for (int k = 0; k < 9; k++)
+  double *state[10] = {h->x, h->v, h->a, h->du, h->d, h->S, h->evp, h->p, (double *)h->st, h->T};
+  const long long off[10] = {L.x, L.v, L.a, L.du, L.d, L.S, L.evp, L.p, L.st, L.T};
+  const long long cnt[10] = {n3, n3, n3, n3, n3, 9 * nipt, nipt, h->ne, (long long)(sizeof(StepState) / 8), nipt};
+  const int count = L.T >= 0 ? 10 : 9;
+  const double *src[10];
+  double *dst[10];
+  for (int k = 0; k < count; k++)
   {
     src[k] = to_snap ? state[k] : snap + off[k];
     dst[k] = to_snap ? snap + off[k] : state[k];
   }
-  launch_copy_list(src, dst, cnt, 9, h->stream);
+  if (!launch_copy_list(src, dst, cnt, count, h->stream))
+  {
+    h->err = "dnlg_adjoint: snapshot array count out of range";
+    return DNLG_E_ARG;
+  }
   DNLG_HIP(h, hipGetLastError());
   return DNLG_OK;
 }
@@ -73,11 +80,12 @@ static int snap_from(dnlg_handle *h, const Snap &L, const double *src) { return 
 
 static int adj_check(dnlg_handle *h)
 {
-  if (h->dim != DNLG_DIM_AXI || h->form != DNLG_FORM_FP64 || h->ea.law != 1 || h->has_bcs ||
+  const bool law_ok = h->ea.law == 1 || (h->ea.law == 2 && h->ea.C == 0.0 && h->T);
+  if (h->dim != DNLG_DIM_AXI || h->form != DNLG_FORM_FP64 || !law_ok || h->has_bcs ||
       h->ts_method != DNLG_TS_COURANT)
   {
-    h->err = "dnlg_adjoint: axisymmetric mesh, FP64 form, hardening A + B evp^n (law 1), no speed BC, "
-             "Courant time step";
+    h->err = "dnlg_adjoint: axisymmetric mesh, FP64 form, hardening A + B evp^n (law 1) or Johnson-Cook with "
+             "the temperature factor and no rate term (law 2, C = 0), no speed BC, Courant time step";
     return DNLG_E_ARG;
   }
   return DNLG_OK;
@@ -120,7 +128,9 @@ extern "C" int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_e
   L.evp = L.S + 9 * nipt;
   L.p = L.evp + nipt;
   L.st = L.p + h->ne;
-  L.size = L.st + (long long)((sizeof(StepState) + 7) / 8);
+  const bool jc = h->ea.law == 2;
+  L.T = jc ? L.st + (long long)((sizeof(StepState) + 7) / 8) : -1;
+  L.size = jc ? L.T + nipt : L.st + (long long)((sizeof(StepState) + 7) / 8);
   const int nbn = (int)((h->nn + 255) / 256), nbe = (int)((nipt + 63) / 64);
   const long long nslot = (long long)h->maxdeg * h->nn * 2;
   h->adj = A;
@@ -148,6 +158,7 @@ extern "C" int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_e
   ACK(dalloc(&A->Sb, 9 * nipt));
   ACK(dalloc(&A->evb, nipt));
   ACK(dalloc(&A->pb, h->ne));
+  if (jc) ACK(dalloc(&A->Tb, nipt));
   ACK(dalloc(&A->slot_x, nslot));
   ACK(dalloc(&A->slot_u, nslot));
   ACK(dalloc(&A->dt_part, nbn));
@@ -155,7 +166,7 @@ extern "C" int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_e
   ACK(dalloc(&A->sc, 1));
 #undef ACK
   A->bytes = sizeof(double) * ((size_t)L.size * (A->nseg + (A->every > 1 ? A->every + 1 : 0) + 1) + h->nn * 10 +
-                               11 * nipt + h->ne + 2 * nslot + nbn + 3 * nbe) + sizeof(AdjScalars);
+                               (jc ? 12 : 11) * nipt + h->ne + 2 * nslot + nbn + 3 * nbe) + sizeof(AdjScalars);
   hipEvent_t e0, e1;
   DNLG_HIP(h, hipEventCreate(&e0));
   DNLG_HIP(h, hipEventCreate(&e1));
@@ -215,6 +226,7 @@ extern "C" int dnlg_adjoint_backward(dnlg_handle *h, const double *xbar, double 
   DNLG_HIP(h, hipMemsetAsync(A->Sb, 0, sizeof(double) * 9 * nipt, s));
   DNLG_HIP(h, hipMemsetAsync(A->evb, 0, sizeof(double) * nipt, s));
   DNLG_HIP(h, hipMemsetAsync(A->pb, 0, sizeof(double) * h->ne, s));
+  if (A->Tb) DNLG_HIP(h, hipMemsetAsync(A->Tb, 0, sizeof(double) * nipt, s));
   DNLG_HIP(h, hipMemsetAsync(A->sc, 0, sizeof(AdjScalars), s));
   StepState s0;
   DNLG_HIP(h, hipMemcpyAsync(&s0, A->ckpt + L.st, sizeof s0, hipMemcpyDeviceToHost, s));
@@ -234,6 +246,8 @@ extern "C" int dnlg_adjoint_backward(dnlg_handle *h, const double *xbar, double 
   g.Sb = A->Sb;
   g.evb = A->evb;
   g.pb = A->pb;
+  g.jc = L.T >= 0 ? 1 : 0;
+  g.Tb = A->Tb;
   g.fb = A->fb;
   g.slot_x = A->slot_x;
   g.slot_u = A->slot_u;
@@ -270,6 +284,7 @@ extern "C" int dnlg_adjoint_backward(dnlg_handle *h, const double *xbar, double 
     g.S = t0 + L.S;
     g.evp = t0 + L.evp;
     g.p = t0 + L.p;
+    g.T = L.T >= 0 ? t0 + L.T : nullptr;
     g.a1 = t1 + L.a;
     g.st0 = (const StepState *)(t0 + L.st);
     g.st1 = (const StepState *)(t1 + L.st);

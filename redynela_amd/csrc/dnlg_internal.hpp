@@ -154,6 +154,12 @@ struct AdjArgs
   double alphaM, beta, gamma, safety, omegaS, cs;
   int freq;
   long long k;                      // this launch reverses increment k+1
+  // Johnson-Cook temperature factor and adiabatic heating (law 2 without the
+  // rate term): point temperature after increment k and its adjoint; the
+  // factor's constants are ea.T0, ea.Tm, ea.m, ea.chi_rho_cp
+  int jc;
+  const double *T;                  // [4 ne] or null
+  double *Tb;                       // [4 ne] or null
 };
 
 }  // namespace dnl

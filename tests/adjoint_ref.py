@@ -4,7 +4,12 @@ adjoint w.r.t. (A, B, n) -- test infrastructure for the GPU adjoint.
 The forward is the smooth idealisation of the reference's increment
 (FE/ExplicitSolverCH.C:115-278, FE/Element.C:164-403): closed-form 2x2 polar
 decomposition instead of the Jacobi iteration, radial return converged to
-round-off instead of the 1e-8 stop.  Its adjoint is written out step by step
+round-off instead of the 1e-8 stop.  With thermal=dict(m, T0, Tm, cp, tq) the
+yield stress carries the Johnson-Cook temperature factor of
+FE/IsoHardJohnsonCook.h:51-68, sigma_y = (A + B evp^n)(1 - T*^m), with
+T* = (T - T0)/(Tm - T0) at the point's temperature before the increment, and
+the point heats adiabatically by tq sigma_y d(evp) / (rho cp) after the
+return (no rate term).  Its adjoint is written out step by step
 (nodal update, assembly, element kinematics, polar decomposition, return
 mapping, time-step control) and is checked against central differences of this
 forward, which pins the derivation independently of the GPU code.
@@ -34,7 +39,7 @@ for pt in range(4):
 
 
 class Model:
-    def __init__(self, mesh, E=115e9, nu=0.3, rho=8930.0, rho_b=1.0, safety=0.9, freq=10):
+    def __init__(self, mesh, E=115e9, nu=0.3, rho=8930.0, rho_b=1.0, safety=0.9, freq=10, thermal=None):
         self.x0 = mesh["x"][:, :2].copy()
         self.conn = mesh["conn"].astype(np.int64)
         self.bc = mesh["bc"]
@@ -52,6 +57,21 @@ class Model:
         # restrained masks per comp
         self.fix = np.stack([(self.bc & 1) > 0, (self.bc & 2) > 0], 1)
         self.mass = self.lumped_mass(self.x0)
+        self.thermal = thermal
+        if thermal:
+            self.T0, self.Tm, self.mexp = thermal["T0"], thermal["Tm"], thermal["m"]
+            self.kheat = thermal["tq"] / (rho * thermal["cp"])
+
+    def theta(self, T):
+        """temperature factor and its derivative"""
+        if not self.thermal:
+            return np.ones_like(T), np.zeros_like(T)
+        ts = (T - self.T0) / (self.Tm - self.T0)
+        inside = (ts > 0) & (ts < 1)
+        tsc = np.where(inside, ts, 0.5)
+        th = np.where(ts <= 0, 1.0, np.where(ts >= 1, 0.0, 1 - tsc ** self.mexp))
+        dth = np.where(inside, -self.mexp * tsc ** (self.mexp - 1) / (self.Tm - self.T0), 0.0)
+        return th, dth
 
     def geom(self, x):
         X = x[self.conn]  # (ne,4,2)
@@ -101,8 +121,11 @@ class Model:
         return g / self.cs
 
     # ---------------- element ----------------
-    def elem_forward(self, x, du, S, evp, p, th, keep=False):
+    def elem_forward(self, x, du, S, evp, p, th, T=None, keep=False):
         A, B, n = th
+        if T is None:
+            T = np.zeros_like(evp)
+        tf, dtf = self.theta(T)
         J, det, inv, Rr, d = self.geom(x)
         U = du[self.conn]  # (ne,4,2)
         ne = len(self.conn)
@@ -132,25 +155,28 @@ class Model:
         St = S + 2 * self.Gm * dev
         s_ = np.sqrt(np.sum(St ** 2, axis=(-1, -2)))
         Strial = S32 * s_
-        sig0 = A + B * evp ** n
+        sig0 = (A + B * evp ** n) * tf
         plastic = Strial > sig0
         gam = np.zeros_like(evp)
         if plastic.any():
             g = np.zeros(plastic.sum())
-            ev0 = evp[plastic]; St_ = Strial[plastic]
+            ev0 = evp[plastic]; St_ = Strial[plastic]; tp = tf[plastic]
             for it in range(100):
                 ev = ev0 + S23 * g
-                phi = S32 * 2 * self.Gm * g + A + B * ev ** n - St_
-                dphi = S32 * 2 * self.Gm + S23 * n * B * ev ** (n - 1)
+                phi = S32 * 2 * self.Gm * g + (A + B * ev ** n) * tp - St_
+                dphi = S32 * 2 * self.Gm + S23 * n * B * ev ** (n - 1) * tp
                 dg = -phi / dphi
                 dg = np.where((dg < 0) & (-dg > g), -0.9 * g, dg)
                 g = g + dg
-                if np.all(np.abs(phi) <= 1e-15 * (A + B * ev ** n)):
+                if np.all(np.abs(phi) <= 1e-15 * (A + B * ev ** n) * tp):
                     break
             gam[plastic] = g
         cfac = np.where(plastic, 1 - 2 * self.Gm * gam / np.where(s_ > 0, s_, 1), 1.0)
         Sp = cfac[..., None, None] * St
         evp_new = evp + S23 * gam
+        T_new = T.copy()
+        if self.thermal:
+            T_new = np.where(plastic, T + self.kheat * (A + B * evp_new ** n) * tf * S23 * gam, T)
         pres = np.mean(tr, axis=1)
         p_new = p + self.K * pres
         Srot = np.einsum('...ij,...jk,...lk->...il', Rm, Sp, Rm)
@@ -163,18 +189,23 @@ class Model:
             - np.einsum('qL,eq->eL', N, Sig[..., 2, 2] * W / Rr)
         fe[..., 1] = -np.einsum('eqL,eq->eL', d[..., 0], Sig[..., 0, 1] * W) \
             - np.einsum('eqL,eq->eL', d[..., 1], Sig[..., 1, 1] * W)
-        out = (Srot, evp_new, p_new, fe)
+        out = (Srot, evp_new, p_new, fe, T_new)
         if keep:
             tape = dict(J=J, det=det, inv=inv, Rr=Rr, d=d, U=U, F=F, vr=vr, a_=a_, b_=b_, rr=rr, c=c, s=s, Rm=Rm,
                         U2=U2, lam=lam, Q=Q, dE=dE, St=St, s_=s_, Strial=Strial, plastic=plastic, gam=gam,
-                        cfac=cfac, Sp=Sp, Sig=Sig, W=W, evp=evp)
+                        cfac=cfac, Sp=Sp, Sig=Sig, W=W, evp=evp, tf=tf, dtf=dtf)
             return out, tape
         return out
 
-    def elem_backward(self, x, du, S, evp, p, th, fbar, Sbar, evbar, pbar):
-        """VJP of elem_forward: returns xbar (nn,2), dubar (nn,2), Sbar_in, evbar_in, pbar_in, thbar (3)."""
+    def elem_backward(self, x, du, S, evp, p, th, fbar, Sbar, evbar, pbar, Temp=None, Tbar=None):
+        """VJP of elem_forward: returns xbar (nn,2), dubar (nn,2), Sbar_in, evbar_in, pbar_in, thbar (3),
+        Tbar_in."""
         A, B, n = th
-        (Srot, evp_new, p_new, fe), T = self.elem_forward(x, du, S, evp, p, th, keep=True)
+        if Temp is None:
+            Temp = np.zeros_like(evp)
+        if Tbar is None:
+            Tbar = np.zeros_like(evp)
+        (Srot, evp_new, p_new, fe, _), T = self.elem_forward(x, du, S, evp, p, th, Temp, keep=True)
         d, W, Rr, Sig, Rm, Sp = T['d'], T['W'], T['Rr'], T['Sig'], T['Rm'], T['Sp']
         ne = len(self.conn)
         Sigb = np.zeros((ne, 4, 3, 3))
@@ -207,18 +238,36 @@ class Model:
         evb_in = evbar.copy()
         Stb = cfac[..., None, None] * Spb
         cb = np.sum(Spb * St, axis=(-1, -2))
+        tf, dtf = T['tf'], T['dtf']
+        Tb_in = Tbar.copy()
         gb = np.where(plastic, -2 * self.Gm / np.where(s_ > 0, s_, 1) * cb + S23 * evbar, 0.0)
         sb = np.where(plastic, 2 * self.Gm * gam / np.where(s_ > 0, s_, 1) ** 2 * cb, 0.0)
         ev = evp + S23 * gam
         with np.errstate(all='ignore'):
-            hp = n * B * ev ** (n - 1)
-            dphi = S32 * 2 * self.Gm + S23 * hp
+            evn = ev ** n
+            base = A + B * evn
+            hb = n * B * ev ** (n - 1)
+            lev = np.log(ev)
+            thA = np.zeros_like(evp); thB = np.zeros_like(evp); thN = np.zeros_like(evp)
+            if self.thermal:
+                # T' = T + k base(ev) theta(T) sqrt(2/3) gamma on the plastic branch
+                kq = np.where(plastic, Tbar * self.kheat, 0.0)
+                dep = S23 * gam
+                gb = gb + np.where(plastic, kq * tf * S23 * (hb * dep + base), 0.0)
+                evb_in += np.where(plastic, kq * tf * hb * dep, 0.0)
+                Tb_in += np.where(plastic, kq * base * dtf * dep, 0.0)
+                thA += np.where(plastic, kq * tf * dep, 0.0)
+                thB += np.where(plastic, kq * tf * dep * evn, 0.0)
+                thN += np.where(plastic, kq * tf * dep * B * evn * lev, 0.0)
+            dphi = S32 * 2 * self.Gm + S23 * hb * tf
             w = np.where(plastic, gb / dphi, 0.0)
             Strialb = w
-            evb_in += np.where(plastic, -w * hp, 0.0)
-            thb = np.array([np.sum(np.where(plastic, -w, 0.0)),
-                            np.sum(np.where(plastic, -w * ev ** n, 0.0)),
-                            np.sum(np.where(plastic, -w * B * ev ** n * np.log(ev), 0.0))])
+            evb_in += np.where(plastic, -w * hb * tf, 0.0)
+            Tb_in += np.where(plastic, -w * base * dtf, 0.0)
+            thA += np.where(plastic, -w * tf, 0.0)
+            thB += np.where(plastic, -w * tf * evn, 0.0)
+            thN += np.where(plastic, -w * tf * B * evn * lev, 0.0)
+            thb = np.array([np.sum(thA), np.sum(thB), np.sum(thN)])
         sb = sb + S32 * Strialb
         Stb = Stb + np.where(plastic, sb / np.where(s_ > 0, s_, 1), 0.0)[..., None, None] * St
         Sb_in = Stb.copy()
@@ -271,7 +320,7 @@ class Model:
         xb = np.zeros((len(x), 2)); dub = np.zeros((len(x), 2))
         np.add.at(xb, self.conn, Xb)
         np.add.at(dub, self.conn, Ub)
-        return xb, dub, Sb_in, evb_in, pb_in, thb
+        return xb, dub, Sb_in, evb_in, pb_in, thb, Tb_in
 
     # ---------------- nodal ----------------
     def nodal(self, x, v, a, f, dt):
@@ -305,6 +354,7 @@ class Model:
         x = self.x0.copy(); v = self.v0.copy(); a = np.zeros_like(x); du = np.zeros_like(x)
         ne = len(self.conn)
         S = np.zeros((ne, 4, 3, 3)); evp = np.full((ne, 4), 1e-10); p = np.zeros(ne)
+        Temp = np.full((ne, 4), self.T0 if self.thermal else 0.0)
         t = 0.0
         crit = self.courant(x)
         dt = self.safety * self.om / (2 / crit.min())
@@ -322,8 +372,8 @@ class Model:
                 elif ts != lts:
                     dx = (t - lts) / (ts - lts)
                     dt = dx * Cc + (1 - dx) * Ls
-            state = (x, v, a, du, S, evp, p)
-            S, evp, p, fe = self.elem_forward(x, du, S, evp, p, th)
+            state = (x, v, a, du, S, evp, p, Temp)
+            S, evp, p, fe, Temp = self.elem_forward(x, du, S, evp, p, th, Temp)
             f = np.zeros_like(x)
             np.add.at(f, self.conn, fe)
             x, v, a, du = self.nodal(x, v, a, f, dt)
@@ -337,17 +387,17 @@ class Model:
         xN, hist = self.run(th, nsteps, tape=True)
         Xb = 2 * (xN - xt); Ub = np.zeros_like(Xb); Vb = np.zeros_like(Xb); Ab = np.zeros_like(Xb)
         ne = len(self.conn)
-        Sb = np.zeros((ne, 4, 3, 3)); evb = np.zeros((ne, 4)); pb = np.zeros(ne)
+        Sb = np.zeros((ne, 4, 3, 3)); evb = np.zeros((ne, 4)); pb = np.zeros(ne); Tb = np.zeros((ne, 4))
         thb = np.zeros(3)
         tb = 0.0; dtb_carry = 0.0; tsb = ltsb = Cb = Lb = 0.0
         for k in range(nsteps - 1, -1, -1):
-            (x, v, a, du, S, evp, p), sc, a_new = hist[k]
+            (x, v, a, du, S, evp, p, Temp), sc, a_new = hist[k]
             dt = sc['dt']
             dtb = tb + dtb_carry  # t_{k+1} = t_k + dt ; dt carried from next step if unchanged
             xb, vb, ab, fb, dtb_n = self.nodal_backward(v, a, a_new, dt, Xb, Ub, Vb, Ab)
             dtb += dtb_n
             fbar = fb[self.conn]
-            xb2, dub, Sb, evb, pb, thb_e = self.elem_backward(x, du, S, evp, p, th, fbar, Sb, evb, pb)
+            xb2, dub, Sb, evb, pb, thb_e, Tb = self.elem_backward(x, du, S, evp, p, th, fbar, Sb, evb, pb, Temp, Tb)
             thb += thb_e
             Xb = xb + xb2; Ub = dub; Vb = vb; Ab = ab
             # time-step computation of this increment

@@ -32,3 +32,27 @@ def test_restrained_nodes_and_zero_seed_give_zero_gradient():
     xN, _ = m.run(th, 30)
     _, g = m.grad(th, 30, xN)  # loss at its minimum: zero seed
     assert np.all(g == 0.0)
+
+
+def test_thermal_adjoint_formulas_match_central_differences():
+    """the Johnson-Cook temperature factor and adiabatic heating (no rate
+    term): the point temperature's adjoint and the heating's dependence on
+    (A, B, n) against central differences of the same forward"""
+    import redynela_amd as R
+    mesh = R.mesh.taylor_axi(3, 12, 3.2e-3, 3.888e-3, 250.0)
+    th_m = dict(m=1.09, T0=293.0, Tm=1356.0, cp=383.0, tq=0.9)
+    m = AR.Model(mesh, thermal=th_m)
+    th0 = np.array([90e6, 292e6, 0.31])
+    nst = 120
+    xt, _ = m.run(th0, nst)
+    # the heating is felt: the same run without it ends elsewhere
+    assert np.max(np.abs(AR.Model(mesh).run(th0, nst)[0] - xt)) > 1e-9
+    th = th0 * np.array([1.1, 0.85, 1.15])
+    xN, g = m.grad(th, nst, xt)
+    for i in range(3):
+        h = 1e-5
+        tp, tm = th.copy(), th.copy()
+        tp[i] *= 1 + h
+        tm[i] *= 1 - h
+        fd = (np.sum((m.run(tp, nst)[0] - xt) ** 2) - np.sum((m.run(tm, nst)[0] - xt) ** 2)) / (2 * h * th[i])
+        assert abs(fd - g[i]) <= 1e-7 * abs(fd), (i, fd, g[i])

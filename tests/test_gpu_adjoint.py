@@ -9,7 +9,13 @@
   restatement of the reference solver): within 1e-5 at relative step 1e-6
   once the solver's stops are tightened, and within its noise floor (2e-4)
   with the reference's own stops;
-* the torch.autograd.Function returns the same gradient.
+* the torch.autograd.Function returns the same gradient;
+* with the Johnson-Cook temperature factor and adiabatic heating (the
+  precision workload's m, T0, Tm, cp and Taylor-Quinney fraction, no rate
+  term) the gradient equals the numpy restatement's and, on the 20 x 100
+  workload, central differences of the oracle's same law within 1e-5 (stops
+  tightened); the reference evaluates A + B evp^n only, so this part has no
+  reference counterpart.
 """
 import sys
 import time
@@ -28,7 +34,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-def oracle_final(mesh, theta, nsteps, smooth=False):
+def oracle_final(mesh, theta, nsteps, smooth=False, thermal=None):
     """Final (r, z) of the oracle run.  smooth: test-only tighter stops of the
     radial return (1e-13 instead of 1e-8) and of the Jacobi polar iteration
     (1e-15 instead of 1e-9), which take the reference's iteration-count jumps
@@ -37,7 +43,11 @@ def oracle_final(mesh, theta, nsteps, smooth=False):
     sol = dict(rho_b=1.0)
     if smooth:
         sol["rr_prec"] = 1e-13
-    m = O.Model(mesh, dict(E=115e9, nu=0.3, rho=8930.0, A=A, B=B, n=n), sol, mathmode=1)
+    mat = dict(E=115e9, nu=0.3, rho=8930.0, A=A, B=B, n=n)
+    if thermal:
+        mat["jc"] = dict(C=0.0, epsd0=1.0, m=thermal["m"], T0=thermal["T0"], Tm=thermal["Tm"], cp=thermal["cp"],
+                         tq=thermal["tq"])
+    m = O.Model(mesh, mat, sol, mathmode=1)
     O.lib().dnlo_set_polar_precision(1e-15 if smooth else 10.0e-10)
     try:
         m.begin(1e300)
@@ -48,11 +58,12 @@ def oracle_final(mesh, theta, nsteps, smooth=False):
     return m.get(O.F_X).reshape(-1, 3)[:, :2]
 
 
-def central(mesh, th, i, h, nst, loss, smooth=False):
+def central(mesh, th, i, h, nst, loss, smooth=False, thermal=None):
     tp, tm = th.copy(), th.copy()
     tp[i] *= 1 + h
     tm[i] *= 1 - h
-    return (loss(oracle_final(mesh, tp, nst, smooth)) - loss(oracle_final(mesh, tm, nst, smooth))) / (2 * h * th[i])
+    return (loss(oracle_final(mesh, tp, nst, smooth, thermal)) - loss(oracle_final(mesh, tm, nst, smooth, thermal))) \
+        / (2 * h * th[i])
 
 
 def test_adjoint_forward_is_the_production_stepping():
@@ -180,3 +191,73 @@ def test_adjoint_checkpoint_every_increment_needs_no_recompute():
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert out[1][2]["recompute_ms"] == 0.0 and out[1][2]["n_checkpoints"] == 150
+
+
+def test_thermal_adjoint_equals_numpy_restatement():
+    import redynela_amd as R
+    from redynela_amd import adjoint as AD
+    say("case: axi 4x16, 250 m/s, JC temperature factor + heating, 230 increments, checkpoints every 60, "
+        "GPU vs numpy adjoint")
+    mesh = R.mesh.taylor_axi(4, 16, 3.2e-3, 5.2e-3, 250.0)
+    th0 = np.array([90e6, 292e6, 0.31])
+    nst = 230
+    m = AR.Model(mesh, thermal=AD.THERMAL)
+    xt, _ = m.run(th0, nst)
+    th = th0 * np.array([1.12, 0.9, 1.1])
+    xr, gr = m.grad(th, nst, xt)
+    p = AD.HardeningAdjoint(mesh, nsteps=nst, ckpt_every=60, thermal=AD.THERMAL)
+    x = p.forward(th)[:, :2]
+    g = p.backward(2 * (x - xt))
+    say("  numpy", gr, "\n  gpu  ", g, "\n  rel  ", np.abs(g - gr) / np.abs(gr))
+    assert np.max(np.abs(x - xr)) < 1e-10 * np.max(np.abs(xr))
+    assert np.all(np.abs(g - gr) <= 1e-7 * np.abs(gr))
+    # the temperature factor matters: the plain law's gradient is another one
+    p2 = AD.HardeningAdjoint(mesh, nsteps=nst, ckpt_every=60)
+    x2 = p2.forward(th)[:, :2]
+    g2 = p2.backward(2 * (x2 - xt))
+    assert np.max(np.abs(g2 - g) / np.abs(g)) > 1e-4
+    p.close()
+    p2.close()
+
+
+def test_thermal_adjoint_bar_20x100_matches_oracle_central_differences():
+    from redynela_amd import adjoint as AD
+    mesh = AD.bar_20x100()
+    nst = AD.ADJ_STEPS
+    say("case: axi 20x100, 200 m/s, JC temperature factor + heating (m 1.09, T0 293 K, Tm 1356 K, cp 383, "
+        "chi 0.9), %d increments, checkpoints every %d" % (nst, AD.ADJ_CKPT))
+    T = AD.THERMAL
+    th = AD.perturbed_start()
+    p = AD.HardeningAdjoint(mesh, nsteps=nst, ckpt_every=AD.ADJ_CKPT, thermal=T)
+    x = p.forward(th)[:, :2]
+    xo = oracle_final(mesh, th, nst, thermal=T)
+    assert np.array_equal(x, xo)  # the forward is the oracle's same law, bit for bit
+    xts = oracle_final(mesh, AD.TARGET_THETA, nst, smooth=True, thermal=T)
+    g = p.backward(2 * (x - xts))
+    st = p.stats
+    say("  theta", th, "\n  adjoint grad", g)
+    say("  forward %.1f ms, backward %.1f ms, device memory %.1f MB" % (st["forward_ms"], st["backward_ms"],
+                                                                     st["device_bytes"] / 1e6))
+    loss_s = lambda xx: np.sum((xx - xts) ** 2)  # noqa: E731
+    for i in range(3):
+        fd = central(mesh, th, i, 1e-6, nst, loss_s, smooth=True, thermal=T)
+        rel = abs(fd - g[i]) / abs(fd)
+        say("  smoothed FD, param %d: h=1e-06 fd=%.9e rel=%.2e" % (i, fd, rel))
+        assert rel <= 1e-5
+    p.close()
+
+
+def test_adjoint_refuses_the_rate_term():
+    import redynela_amd as R
+    from redynela_amd import _lib as L
+    from redynela_amd._lib import DnlgError
+    mesh = R.mesh.taylor_axi(2, 6, 3.2e-3, 1.9e-3, 150.0)
+    h = R.IsoHardJohnsonCook(90e6, 292e6, 0.31, C=0.025, m=1.09, Tm=1356.0, T0=293.0)
+    h.enableRateAndThermal(383.0, 0.9)
+    s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
+    s.setDissipation(1.0)
+    d = R.Domain(mesh, R.Material(115e9, 0.3, 8930.0, h), s)
+    with pytest.raises(DnlgError):
+        L.check(L.lib().dnlg_adjoint_forward(d.h, 10, 5), d.h)
+    d.close()
