@@ -119,7 +119,7 @@ def main():
         shard = shard_leg(args)
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_leg()
+        adjoint = adjoint_leg(mesh)
     pulse = None
     if not args.no_pulse:
         pulse = pulse_leg(args, elapsed / args.steps * 1e3)
@@ -375,10 +375,12 @@ def pulse_leg(args, courant_ms_per_step, steps=200):
             "dt_s": tm["dt"], "stiffness_bytes": ne * 24 * 24 * 8}
 
 
-def adjoint_leg():
+def adjoint_leg(hex_mesh):
     """Gradient of the final-shape misfit w.r.t. (A, B, n) on the axisymmetric
     20 x 100 bar, 4,000 increments, checkpoints every 100; time and memory of
-    the reverse sweep against one forward run."""
+    the reverse sweep against one forward run.  The same on the 1M hex8 bar
+    (100 increments, checkpoints every 10), timing only: the gradient is
+    checked against the oracle on a small hex8 bar in the tests."""
     import numpy as np
     from redynela_amd import adjoint as AD
     mesh = AD.bar_20x100()
@@ -404,6 +406,12 @@ def adjoint_leg():
     gr = r.backward(2 * (xr - xr_t))
     sr = r.stats
     r.close()
+    hx = AD.HardeningAdjoint(hex_mesh, nsteps=100, ckpt_every=10)
+    xh_t = hx.forward(AD.TARGET_THETA).copy()
+    xh = hx.forward(AD.perturbed_start())
+    gh = hx.backward(2 * (xh - xh_t))
+    sh = hx.stats
+    hx.close()
     return {"workload": "axi 20x100 quads, 200 m/s, 4000 increments, checkpoints every 100, FP64",
             "grad": [float(v) for v in g], "forward_ms": st["forward_ms"], "backward_ms": st["backward_ms"],
             "recompute_ms": st["recompute_ms"], "backward_over_forward": st["backward_ms"] / st["forward_ms"],
@@ -416,7 +424,13 @@ def adjoint_leg():
                                "Tm 1356 K, cp 383 J/kg/K, Taylor-Quinney 0.9, no rate term",
                         "grad": [float(v) for v in gr], "forward_ms": sr["forward_ms"],
                         "backward_ms": sr["backward_ms"], "backward_over_forward": sr["backward_ms"] / sr["forward_ms"],
-                        "device_MB": sr["device_bytes"] / 1e6}}
+                        "device_MB": sr["device_bytes"] / 1e6},
+            "hex8": {"workload": "hex8 O-grid, %d elements, 227 m/s, power law (A, B, n), 100 increments, "
+                                 "checkpoints every 10, FP64" % hex_mesh["conn"].shape[0],
+                     "grad": [float(v) for v in gh], "forward_ms": sh["forward_ms"],
+                     "backward_ms": sh["backward_ms"], "recompute_ms": sh["recompute_ms"],
+                     "backward_over_forward": sh["backward_ms"] / sh["forward_ms"],
+                     "device_MB": sh["device_bytes"] / 1e6, "checkpoint_MB": sh["checkpoint_bytes"] / 1e6}}
 
 
 def hex_lengths(x, conn, chunk=262144):

@@ -353,18 +353,21 @@ const char *dnlg_batch_last_error(dnlg_batch *b);
 int dnlg_batch_taylor_params(int n, unsigned long long seed, int *nr, double *v0, double *sy0, double *hmod);
 
 /* ---- adjoint: gradient of a function of the final coordinates w.r.t. (A, B, n) ----
- * No reference counterpart.  For an axisymmetric FP64 handle with hardening
- * A + B evp^n (and no speed BC) that has not stepped yet:
+ * No reference counterpart.  For an axisymmetric or hex8 FP64 handle with
+ * hardening A + B evp^n, or the Johnson-Cook law with its temperature factor
+ * and adiabatic heating but no rate term (law 2, C = 0), no speed BC, Courant
+ * time step, that has not stepped yet:
  * dnlg_adjoint_forward runs the solve() preamble and nsteps increments
  * (the same bits as dnlg_begin + dnlg_step) and keeps the state after every
  * ckpt_every-th increment in device memory; dnlg_adjoint_backward takes the
- * adjoint xbar ([n_nodes][3], the third component ignored) of the final node
- * coordinates and returns grad = (dL/dA, dL/dB, dL/dn).  The reverse sweep
- * restores each checkpoint, recomputes its segment and reverses it: the
- * Chung-Hulbert update, the restraints, the element-order assembly, the
- * element kinematics (closed-form 2x2 polar decomposition), the radial return
- * with its elastic/plastic switch, and the time step (Courant arg-min element,
- * LINEARdbl extrapolation).  With ckpt_every = 1 every increment's state is
+ * adjoint xbar ([n_nodes][3]; axisymmetric: the third component ignored) of
+ * the final node coordinates and returns grad = (dL/dA, dL/dB, dL/dn).  The
+ * reverse sweep restores each checkpoint, recomputes its segment and
+ * reverses it: the Chung-Hulbert update, the restraints, the element-order
+ * assembly, the element kinematics (closed-form 2x2 polar decomposition;
+ * in 3D the exact eigen-decomposition of F^T F), the radial return with its
+ * elastic/plastic switch (and the point temperature), and the time step
+ * (Courant arg-min element, LINEARdbl extrapolation).  With ckpt_every = 1 every increment's state is
  * kept (nsteps snapshots) and the sweep reverses over them with no
  * recomputation: memory for time, the same gradient bit for bit.  After it
  * the handle holds the post-forward state. */

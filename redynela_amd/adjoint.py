@@ -1,4 +1,4 @@
-"""Adjoint of the axisymmetric explicit solve: gradients of a function of the
+"""Adjoint of the axisymmetric or hex8 explicit solve: gradients of a function of the
 final nodal coordinates with respect to the hardening parameters (A, B, n) of
 sigma_y = A + B evp^n -- or of the Johnson-Cook law with its temperature
 factor and adiabatic heating but no rate term, sigma_y = (A + B evp^n)
@@ -6,7 +6,7 @@ factor and adiabatic heating but no rate term, sigma_y = (A + B evp^n)
 (dnlg_adjoint_forward / dnlg_adjoint_backward) with checkpoint-recompute.
 
 FinalShape is the torch.autograd.Function over it: final (r, z) of every
-node as a function of theta = (A, B, n).
+node of an axisymmetric mesh as a function of theta = (A, B, n).
 """
 import ctypes as C
 

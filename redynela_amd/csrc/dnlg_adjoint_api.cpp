@@ -34,6 +34,8 @@ struct AdjStore
   double *xb = nullptr, *ub = nullptr, *vb = nullptr, *ab = nullptr, *Sb = nullptr, *evb = nullptr, *pb = nullptr;
   double *Tb = nullptr;  // adjoint of the point temperature (Johnson-Cook temperature factor)
   double *fb = nullptr, *slot_x = nullptr, *slot_u = nullptr, *dt_part = nullptr, *th_part = nullptr;
+  double *am_len = nullptr;
+  long long *am_idx = nullptr;
   AdjScalars *sc = nullptr;
   float fwd_ms = 0.f;
   size_t bytes = 0;
@@ -43,7 +45,7 @@ void adj_free(AdjStore *s)
 {
   if (!s) return;
   void *ptrs[] = {s->ckpt, s->tape, s->fin, s->xb, s->ub, s->vb, s->ab, s->Sb, s->evb, s->pb, s->Tb, s->fb,
-                  s->slot_x, s->slot_u, s->dt_part, s->th_part, s->sc};
+                  s->slot_x, s->slot_u, s->dt_part, s->th_part, s->am_len, s->am_idx, s->sc};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete s;
@@ -81,11 +83,11 @@ static int snap_from(dnlg_handle *h, const Snap &L, const double *src) { return 
 static int adj_check(dnlg_handle *h)
 {
   const bool law_ok = h->ea.law == 1 || (h->ea.law == 2 && h->ea.C == 0.0 && h->T);
-  if (h->dim != DNLG_DIM_AXI || h->form != DNLG_FORM_FP64 || !law_ok || h->has_bcs ||
+  if ((h->dim != DNLG_DIM_AXI && h->dim != DNLG_DIM_3D) || h->form != DNLG_FORM_FP64 || !law_ok || h->has_bcs ||
       h->ts_method != DNLG_TS_COURANT)
   {
-    h->err = "dnlg_adjoint: axisymmetric mesh, FP64 form, hardening A + B evp^n (law 1) or Johnson-Cook with "
-             "the temperature factor and no rate term (law 2, C = 0), no speed BC, Courant time step";
+    h->err = "dnlg_adjoint: axisymmetric or hex8 mesh, FP64 form, hardening A + B evp^n (law 1) or Johnson-Cook "
+             "with the temperature factor and no rate term (law 2, C = 0), no speed BC, Courant time step";
     return DNLG_E_ARG;
   }
   return DNLG_OK;
@@ -132,7 +134,8 @@ extern "C" int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_e
   L.T = jc ? L.st + (long long)((sizeof(StepState) + 7) / 8) : -1;
   L.size = jc ? L.T + nipt : L.st + (long long)((sizeof(StepState) + 7) / 8);
   const int nbn = (int)((h->nn + 255) / 256), nbe = (int)((nipt + 63) / 64);
-  const long long nslot = (long long)h->maxdeg * h->nn * 2;
+  const int dim = h->dim;
+  const long long nslot = (long long)h->maxdeg * h->nn * dim;
   h->adj = A;
 #define ACK(call)                                                                   \
   do                                                                                \
@@ -150,11 +153,11 @@ extern "C" int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_e
   // with a checkpoint after every increment the checkpoints are the tape
   if (A->every > 1) ACK(dalloc(&A->tape, (size_t)L.size * (A->every + 1)));
   ACK(dalloc(&A->fin, (size_t)L.size));
-  ACK(dalloc(&A->xb, h->nn * 2));
-  ACK(dalloc(&A->ub, h->nn * 2));
-  ACK(dalloc(&A->vb, h->nn * 2));
-  ACK(dalloc(&A->ab, h->nn * 2));
-  ACK(dalloc(&A->fb, h->nn * 2));
+  ACK(dalloc(&A->xb, h->nn * dim));
+  ACK(dalloc(&A->ub, h->nn * dim));
+  ACK(dalloc(&A->vb, h->nn * dim));
+  ACK(dalloc(&A->ab, h->nn * dim));
+  ACK(dalloc(&A->fb, h->nn * dim));
   ACK(dalloc(&A->Sb, 9 * nipt));
   ACK(dalloc(&A->evb, nipt));
   ACK(dalloc(&A->pb, h->ne));
@@ -163,10 +166,14 @@ extern "C" int dnlg_adjoint_forward(dnlg_handle *h, long long nsteps, int ckpt_e
   ACK(dalloc(&A->slot_u, nslot));
   ACK(dalloc(&A->dt_part, nbn));
   ACK(dalloc(&A->th_part, 3 * nbe));
+  const int nba = (int)((h->ne + 255) / 256);
+  ACK(dalloc(&A->am_len, nba));
+  ACK(dalloc(&A->am_idx, nba));
   ACK(dalloc(&A->sc, 1));
 #undef ACK
-  A->bytes = sizeof(double) * ((size_t)L.size * (A->nseg + (A->every > 1 ? A->every + 1 : 0) + 1) + h->nn * 10 +
-                               (jc ? 12 : 11) * nipt + h->ne + 2 * nslot + nbn + 3 * nbe) + sizeof(AdjScalars);
+  A->bytes = sizeof(double) * ((size_t)L.size * (A->nseg + (A->every > 1 ? A->every + 1 : 0) + 1) + h->nn * 5 * dim +
+                               (jc ? 11 : 10) * nipt + h->ne + 2 * nslot + nbn + 3 * nbe + nba) +
+             sizeof(long long) * nba + sizeof(AdjScalars);
   hipEvent_t e0, e1;
   DNLG_HIP(h, hipEventCreate(&e0));
   DNLG_HIP(h, hipEventCreate(&e1));
@@ -212,17 +219,15 @@ extern "C" int dnlg_adjoint_backward(dnlg_handle *h, const double *xbar, double 
   const Snap &L = A->lay;
   const long long nn = h->nn, nipt = h->ne * h->nip;
   hipStream_t s = h->stream;
-  // seed: adjoint of the final (r, z) coordinates; everything else zero
-  std::vector<double> seed(nn * 2);
+  // seed: adjoint of the final coordinates ((r, z) or (x, y, z)); everything else zero
+  const int dim = h->dim;
+  std::vector<double> seed(nn * dim);
   for (long long i = 0; i < nn; i++)
-  {
-    seed[2 * i] = xbar[3 * i];
-    seed[2 * i + 1] = xbar[3 * i + 1];
-  }
-  DNLG_HIP(h, hipMemcpyAsync(A->xb, seed.data(), sizeof(double) * nn * 2, hipMemcpyHostToDevice, s));
-  DNLG_HIP(h, hipMemsetAsync(A->ub, 0, sizeof(double) * nn * 2, s));
-  DNLG_HIP(h, hipMemsetAsync(A->vb, 0, sizeof(double) * nn * 2, s));
-  DNLG_HIP(h, hipMemsetAsync(A->ab, 0, sizeof(double) * nn * 2, s));
+    for (int c = 0; c < dim; c++) seed[dim * i + c] = xbar[3 * i + c];
+  DNLG_HIP(h, hipMemcpyAsync(A->xb, seed.data(), sizeof(double) * nn * dim, hipMemcpyHostToDevice, s));
+  DNLG_HIP(h, hipMemsetAsync(A->ub, 0, sizeof(double) * nn * dim, s));
+  DNLG_HIP(h, hipMemsetAsync(A->vb, 0, sizeof(double) * nn * dim, s));
+  DNLG_HIP(h, hipMemsetAsync(A->ab, 0, sizeof(double) * nn * dim, s));
   DNLG_HIP(h, hipMemsetAsync(A->Sb, 0, sizeof(double) * 9 * nipt, s));
   DNLG_HIP(h, hipMemsetAsync(A->evb, 0, sizeof(double) * nipt, s));
   DNLG_HIP(h, hipMemsetAsync(A->pb, 0, sizeof(double) * h->ne, s));
@@ -239,6 +244,7 @@ extern "C" int dnlg_adjoint_backward(dnlg_handle *h, const double *xbar, double 
   g.mass = h->mass;
   g.nn = nn;
   g.ne = h->ne;
+  g.dim = dim;
   g.xb = A->xb;
   g.ub = A->ub;
   g.vb = A->vb;
@@ -255,6 +261,9 @@ extern "C" int dnlg_adjoint_backward(dnlg_handle *h, const double *xbar, double 
   g.th_part = A->th_part;
   g.ndt_part = (int)((nn + 255) / 256);
   g.nth_part = (int)((nipt + 63) / 64);
+  g.am_len = A->am_len;
+  g.am_idx = A->am_idx;
+  g.nam_part = (int)((h->ne + 255) / 256);
   g.sc = A->sc;
   g.K = h->ea.K;
   g.G = h->ea.G;

@@ -139,16 +139,19 @@ struct AdjArgs
   const unsigned char *bc, *deg;
   const double *mass;
   long long nn, ne;
-  // forward tape: state after increment k (x, v, a, du: [nn][3]; S [9][4 ne]; evp [4 ne]; p [ne]) and a_{k+1}
+  // forward tape: state after increment k (x, v, a, du: [nn][3]; S [9][nip ne]; evp [nip ne]; p [ne]) and a_{k+1}
   const double *x, *v, *a, *du, *S, *evp, *p, *a1;
   const StepState *st0, *st1;       // time state after increments k and k+1
   // adjoint state, updated in place from increment k+1 to k
-  double *xb, *ub, *vb, *ab;        // [nn][2]
-  double *Sb, *evb, *pb;            // [9][4 ne], [4 ne], [ne]
-  double *fb;                       // [nn][2] adjoint of the assembled internal force
-  double *slot_x, *slot_u;          // [maxdeg][nn][2] per (element, node) contributions, element order
+  double *xb, *ub, *vb, *ab;        // [nn][dim]
+  double *Sb, *evb, *pb;            // [9][nip ne], [nip ne], [ne]
+  double *fb;                       // [nn][dim] adjoint of the assembled internal force
+  double *slot_x, *slot_u;          // [maxdeg][nn][dim] per (element, node) contributions, element order
   double *dt_part, *th_part;        // per-block partial sums
   int ndt_part, nth_part;
+  double *am_len;                   // per-block Courant arg-min of x_k: length / cs and element
+  long long *am_idx;
+  int nam_part;
   AdjScalars *sc;
   double K, G, A, B, n;
   double alphaM, beta, gamma, safety, omegaS, cs;
@@ -158,8 +161,9 @@ struct AdjArgs
   // rate term): point temperature after increment k and its adjoint; the
   // factor's constants are ea.T0, ea.Tm, ea.m, ea.chi_rho_cp
   int jc;
-  const double *T;                  // [4 ne] or null
-  double *Tb;                       // [4 ne] or null
+  const double *T;                  // [nip ne] or null
+  double *Tb;                       // [nip ne] or null
+  int dim;                          // 2: axisymmetric quad4 (4 points), 3: hex8 (8 points)
 };
 
 }  // namespace dnl
