@@ -303,3 +303,21 @@ def test_hex8_adjoint_matches_oracle_central_differences(law):
     assert np.array_equal(q.backward(2 * (xq - xts)), g)
     p.close()
     q.close()
+
+
+def test_adjoint_refuses_a_finite_end_time():
+    """an end time would stop the loop before nsteps increments and leave
+    no-op increments for the reverse sweep: refused before anything runs"""
+    import redynela_amd as R
+    from redynela_amd import _lib as L
+    from redynela_amd._lib import DnlgError
+    mesh = R.mesh.taylor_axi(2, 6, 3.2e-3, 1.9e-3, 150.0)
+    s = R.ExplicitSolverCH()
+    s.setTimeStepMethod("Courant")
+    s.setDissipation(1.0)
+    s.setTimes(0.0, 1e-7)
+    d = R.Domain(mesh, R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(90e6, 292e6, 0.31)), s)
+    with pytest.raises(DnlgError, match="end_time"):
+        L.check(L.lib().dnlg_adjoint_forward(d.h, 50, 10), d.h)
+    assert d.time()["increment"] == 0
+    d.close()
