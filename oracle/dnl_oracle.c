@@ -74,6 +74,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -532,7 +533,18 @@ typedef struct
   long pulse_iters; /* iterations of the last call */
   double plot_row[4]; /* the endStep history line of the last increment, FE/ExplicitSolverCH.C:459-461 */
   double pulse_conv; /* its last convergence measure */
+  double forces_seconds; /* wall time spent in Domain::computeInternalForces by dnlo_step, the
+                          * region the reference times itself (FE/ExplicitSolverCH.C:475-478) */
 } dnlo;
+
+static double o_wtime(void)
+{
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return (double)clock() / CLOCKS_PER_SEC;
+#endif
+}
 
 static double *o_alloc(size_t n)
 {
@@ -1644,10 +1656,16 @@ int dnlo_step_post(dnlo *m, const double *fint)
 int dnlo_step(dnlo *m)
 {
   dnlo_step_pre(m);
+  const double t0 = o_wtime();
   int st = dnlo_compute_internal_forces(m);
+  m->forces_seconds += o_wtime() - t0;
   if (st) return -st;
   return dnlo_step_post(m, NULL);
 }
+
+/* the accumulated time of the internal-force passes of dnlo_step, and its reset */
+double dnlo_forces_seconds(const dnlo *m) { return m->forces_seconds; }
+void dnlo_reset_forces_seconds(dnlo *m) { m->forces_seconds = 0.0; }
 
 /* ExplicitSolverCH::solve(upTime) in full; returns the number of steps or <0 */
 long dnlo_solve(dnlo *m, double upTime)

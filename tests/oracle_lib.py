@@ -58,6 +58,10 @@ def lib():
         L.dnlo_set_ke.argtypes = [C.c_void_p, C.c_int]
         L.dnlo_set_speed_bc.argtypes = [C.c_void_p, C.c_long, C.c_int, _dp]
         L.dnlo_set_threads.argtypes = [C.c_int]
+        L.dnlo_forces_seconds.restype = C.c_double
+        L.dnlo_forces_seconds.argtypes = [C.c_void_p]
+        L.dnlo_reset_forces_seconds.restype = None
+        L.dnlo_reset_forces_seconds.argtypes = [C.c_void_p]
         L.dnlo_set_strategy.restype = C.c_int
         L.dnlo_set_strategy.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.dnlo_nodal_value.restype = C.c_double
@@ -232,6 +236,13 @@ class Model:
         """Domain::computeInternalForces strategy: 0 (element order) or the reference's case 2, 3, 6, 7"""
         if lib().dnlo_set_strategy(self.h, int(strategy), int(threads)) != 0:
             raise ValueError("oracle: no strategy %r" % (strategy,))
+
+    def forces_seconds(self, reset=False):
+        """wall time of the internal-force passes of step() so far (then reset it)"""
+        v = lib().dnlo_forces_seconds(self.h)
+        if reset:
+            lib().dnlo_reset_forces_seconds(self.h)
+        return v
 
     def jobs(self):
         """(elements in job order, job offsets) of the last Jobs::mapInitial, or None"""
