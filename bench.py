@@ -572,9 +572,10 @@ def cpu_model():
 
 
 def copy_peak():
-    """A plain device copy of 1 GiB (tests/capi/hbm_copy, one kernel, read +
-    write bytes counted), best of 10, timed with HIP events in a child
-    process: the HBM bandwidth a streaming copy reaches on this box."""
+    """A plain device copy of 1 GiB (tests/capi/hbm_copy: three launch shapes
+    of a streaming copy kernel, read + write bytes counted, best of 10 each),
+    timed with HIP events in a child process: the HBM bandwidth a streaming
+    copy reaches on this box."""
     import subprocess
     prog = os.path.join(ROOT, "tests", "capi", "hbm_copy")
     if not os.path.exists(prog):
@@ -587,7 +588,9 @@ def copy_peak():
         return {"error": "hbm_copy exited with %d: %s" % (r.returncode, r.stderr.strip()[-300:])}
     c = json.loads(r.stdout.strip().splitlines()[-1])
     return {"GB_s": c["best_GB_s"], "mean_GB_s": c["mean_GB_s"], "bytes_per_copy": c["bytes_read"] + c["bytes_written"],
-            "how": "tests/capi/hbm_copy: one 16-byte-per-lane copy kernel, 1 GiB read + 1 GiB written"}
+            "kernel": c["kernel"], "blocks_per_cu": c["blocks_per_cu"],
+            "how": "tests/capi/hbm_copy: the fastest of three launch shapes of a 16-byte-per-lane copy kernel "
+                   "(grid-stride, or four loads in flight per lane), 1 GiB read + 1 GiB written"}
 
 
 def cpu_baseline(mesh, args):
