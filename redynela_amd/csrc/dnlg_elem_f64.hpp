@@ -30,6 +30,35 @@ __device__ __forceinline__ void hex_dn0(const ElemArgs &a, int ip, int k, double
   g[1] = sg1(k) > 0 ? m1 : -m1;
   g[2] = sg2(k) > 0 ? m2 : -m2;
 }
+// The hex8 table of dN0(ip, node, d) in the layout of HexLds::tab, built at
+// compile time: the magnitudes are the closed forms (2 - sqrt 3)/12, 1/12 and
+// (2 + sqrt 3)/12 of FE/ElHex8n3D.C:81-87, correctly rounded -- the doubles
+// the host computes for ElemArgs::c0..c2 -- and the signs and the padding
+// after point 3 are those of hex_dn0 and hex_tab.
+constexpr double kHexC0 = 0x1.6dd707e2911cbp-6;
+constexpr double kHexC1 = 0x1.5555555555555p-4;
+constexpr double kHexC2 = 0x1.3e77e4d72c439p-2;
+struct HexTab { double v[8 * 8 * 3 + 4]; };
+constexpr int hex_csg(int k, int d)
+{
+  return d == 0 ? (((k & 3) == 1 || (k & 3) == 2) ? 1 : -1) : (d == 1 ? (((k & 3) >= 2) ? 1 : -1) : (k >= 4 ? 1 : -1));
+}
+constexpr HexTab make_hex_tab()
+{
+  HexTab t{};
+  for (int ip = 0; ip < 8; ip++)
+    for (int k = 0; k < 8; k++)
+      for (int d = 0; d < 3; d++)
+      {
+        const int shared = (hex_csg(ip, (d + 1) % 3) == hex_csg(k, (d + 1) % 3)) +
+                           (hex_csg(ip, (d + 2) % 3) == hex_csg(k, (d + 2) % 3));
+        const double m = shared == 2 ? kHexC2 : (shared == 1 ? kHexC1 : kHexC0);
+        t.v[(ip * 8 + k) * 3 + (ip >= 4 ? 4 : 0) + d] = hex_csg(k, d) > 0 ? m : -m;
+      }
+  return t;
+}
+static __constant__ HexTab kHexTabDev = make_hex_tab();
+
 // quad4 dN0(ip, node, d) = sign(node,d) * (3 +- r3)/12 (FE/ElQua4nAx.C:51-73)
 __device__ __forceinline__ void quad_dn0(const ElemArgs &a, int ip, int k, double *g)
 {
@@ -43,6 +72,21 @@ __device__ __forceinline__ double quad_n(const ElemArgs &a, int ip, int k)
 {
   const int ag = (sg0(ip) == sg0(k)) + (sg1(ip) == sg1(k));
   return ag == 2 ? a.n2 : (ag == 1 ? a.n1 : a.n0);
+}
+
+// a double at byte offset off of a uniform base p, through a raw buffer
+// resource (the base in scalar registers, the 32-bit offset in one vector
+// register; no range clamp: 0x00020000 is the gfx9 descriptor word 3)
+typedef unsigned int dnl_u2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double buf_ld(const double *p, unsigned off)
+{
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(p), 0, -1, 0x00020000);
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_st(double *p, unsigned off, double v)
+{
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(p, 0, -1, 0x00020000);
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(dnl_u2, v), r, off, 0, 0);
 }
 
 __device__ __forceinline__ void raise_status(StepState *st, int code, long long e)
@@ -213,16 +257,12 @@ struct QuadLds
   double x[15][NT + 1];   // rows 2k+d: dN(k, d); 8..12: Sig xx, xy, yx, yy, zz; 13: w detJ; 14: r
 };
 
-// fill the tables (all threads of the block; a barrier must follow)
+// fill the tables (all threads of the block; a barrier must follow); the
+// hex8 table is a copy of the compile-time one
 template <int NT>
-__device__ __forceinline__ void hex_fill_table(const ElemArgs &a, HexLds<NT> &L)
+__device__ __forceinline__ void hex_fill_table(HexLds<NT> &L)
 {
-  for (int i = threadIdx.x; i < 192; i += NT)
-  {
-    double g[3];
-    hex_dn0(a, i / 24, (i / 3) % 8, g);
-    L.tab[hex_tab(i / 24, (i / 3) % 8) + i % 3] = g[i % 3];
-  }
+  for (int i = threadIdx.x; i < 8 * 8 * 3 + 4; i += NT) L.tab[i] = kHexTabDev.v[i];
 }
 template <int NT>
 __device__ __forceinline__ void quad_fill_table(const ElemArgs &a, QuadLds<NT> &L)
@@ -250,22 +290,38 @@ __device__ __forceinline__ void wave_lds_sync()
 // hex8 element kernel: lane = 8*element + integration point
 // ---------------------------------------------------------------------------
 // One integration point of one element per lane; lanes 8k..8k+7 of a wave
-// hold one element.  tid indexes the exchange rows of L (the table must be
-// filled).  JC: the Johnson-Cook law (ElemArgs::law == 2) instead of A + B evp^n.
+// hold one element.  The workgroup's NT lanes take the points blk0 ..
+// blk0 + NT - 1 (blk0 a multiple of NT); tid indexes them and the exchange
+// rows of L (the table must be filled).  JC: the Johnson-Cook law
+// (ElemArgs::law == 2) instead of A + B evp^n.
+//
+// Point and element arrays are addressed from the workgroup's first point,
+// a uniform base that stays in scalar registers, plus one 32-bit lane offset
+// shared by every access, instead of a 64-bit address per array and
+// component in vector registers.
 template <bool SPEED_BC, bool TRACK, bool JC, int NT>
-__device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_raw, long long e_end, int tid,
-                                               HexLds<NT> &Ls)
+__device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long blk0, int tid, HexLds<NT> &Ls)
 {
+  static_assert((NT & (NT - 1)) == 0 && NT % 64 == 0, "whole waves, a power of two");
   StepState *st = a.st;
   const int wl = tid & 63;
   const int pt = wl & 7, base = wl & ~7, lbase = tid & ~7;
-  const bool valid = e_raw < e_end;
-  const long long e = valid ? e_raw : e_end - 1;
-  const long long nipt = a.ne * 8, ip = e * 8 + pt;
+  const long long nipt = a.ne * 8;
+  const long long nvalid = nipt - blk0;  // points of the workgroup that exist: a multiple of 8, at least 8
+  const bool valid = tid < nvalid;
+  // a lane past the end takes the same point of the last element (in bounds,
+  // never written)
+  const unsigned lo = (valid ? (unsigned)tid : (unsigned)(nvalid - 8) + (unsigned)pt) & (unsigned)(NT - 1);
+  const long long e = (blk0 + lo) >> 3;
+  // component c of S through a raw buffer view of the workgroup's points
+  // (base a.S + c * nipt + blk0, in scalar registers), all nine at the one
+  // lane offset: plain pointers would make the compiler fold blk0 + c * nipt
+  // into a 64-bit address per lane and component, held from load to store
+  const unsigned lob = lo * 8u;
   // lane pt of the element fetches local node pt (coordinates and Current
   // increment) and the element's 8 lanes share them through LDS rows 9..14,
   // which hold Sig and w detJ only later in the pass
-  const int mynode = a.conn[e * 8 + pt];
+  const int mynode = (a.conn + blk0)[lo];
   {
     const double *xk = a.x + 3 * (long long)mynode;
     const double *uk = a.du + 3 * (long long)mynode;
@@ -291,8 +347,12 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
   }
   wave_lds_sync();
 
-  // Jacobian at this Gauss point, FE/Element3D.C:75-89
-  T9 J = {{0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  // Jacobian at this Gauss point, FE/Element3D.C:75-89.  The reference's
+  // sums start at +0; these start at node 0's product, which gives the same
+  // double unless all eight products of an entry are -0, i.e. unless all
+  // eight nodes share a zero coordinate -- a flat element, whose singular J
+  // fails the inverse either way.
+  T9 J;
 #pragma unroll
   for (int k = 0; k < 8; k++)
   {
@@ -300,6 +360,13 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
     const double x0 = Ls.x[9][ck], x1 = Ls.x[10][ck], x2 = Ls.x[11][ck];
     const double *tg = Ls.tab + hex_tab(pt, k);
     const double g[3] = {tg[0], tg[1], tg[2]};
+    if (k == 0)
+    {
+      J.v[0] = g[0] * x0; J.v[1] = g[1] * x0; J.v[2] = g[2] * x0;
+      J.v[3] = g[0] * x1; J.v[4] = g[1] * x1; J.v[5] = g[2] * x1;
+      J.v[6] = g[0] * x2; J.v[7] = g[1] * x2; J.v[8] = g[2] * x2;
+      continue;
+    }
     J.v[0] += g[0] * x0; J.v[1] += g[1] * x0; J.v[2] += g[2] * x0;
     J.v[3] += g[0] * x1; J.v[4] += g[1] * x1; J.v[5] += g[2] * x1;
     J.v[6] += g[0] * x2; J.v[7] += g[1] * x2; J.v[8] += g[2] * x2;
@@ -333,20 +400,24 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
 
   T9 S;
 #pragma unroll
-  for (int c = 0; c < 9; c++) S.v[c] = a.S[c * nipt + ip];
-  double evp = a.evp[ip];
+  for (int c = 0; c < 9; c++) S.v[c] = buf_ld(a.S + c * nipt + blk0, lob);
+  double evp = (a.evp + blk0)[lo];
   T9 eps, epl;
   if (TRACK)
   {
 #pragma unroll
-    for (int c = 0; c < 9; c++) { eps.v[c] = a.eps[c * nipt + ip] + dEps.v[c]; epl.v[c] = a.epsplas[c * nipt + ip]; }
+    for (int c = 0; c < 9; c++)
+    {
+      eps.v[c] = (a.eps + blk0 + c * nipt)[lo] + dEps.v[c];
+      epl.v[c] = (a.epsplas + blk0 + c * nipt)[lo];
+    }
   }
   const double tr = dEps.v[0] + dEps.v[4] + dEps.v[8];
   double Tip = 0.0;
   bool rr_ok;
   if (JC)  // law 2, a separate instance: its registers do not weigh on law 1
   {
-    Tip = a.T[ip];
+    Tip = (a.T + blk0)[lo];
     rr_ok = stress_update_jc(a, dEps, S, evp, Tip, st->dt, TRACK ? &epl : nullptr);
   }
   else
@@ -358,7 +429,7 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
 #pragma unroll
   for (int q = 0; q < 8; q++) pressure += __shfl(tr, base + q, 64);
   pressure /= 8;
-  const double p = a.p[e] + a.K * pressure;
+  const double p = (a.p + (blk0 >> 3))[lo >> 3] + a.K * pressure;
 
   // RFI and Sig = S + p I, FE/Element.C:362-401
   S = mul(R, mul_bt(S, R));
@@ -371,17 +442,17 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
   if (valid)
   {
 #pragma unroll
-    for (int c = 0; c < 9; c++) a.S[c * nipt + ip] = S.v[c];
-    a.evp[ip] = evp;
-    if (JC) a.T[ip] = Tip;
-    if (pt == 0) a.p[e] = p;
+    for (int c = 0; c < 9; c++) buf_st(a.S + c * nipt + blk0, lob, S.v[c]);
+    (a.evp + blk0)[lo] = evp;
+    if (JC) (a.T + blk0)[lo] = Tip;
+    if (pt == 0) (a.p + (blk0 >> 3))[lo >> 3] = p;
     if (TRACK)
 #pragma unroll
       for (int c = 0; c < 9; c++)
       {
-        a.eps[c * nipt + ip] = eps.v[c];
-        a.epsplas[c * nipt + ip] = epl.v[c];
-        a.deps[c * nipt + ip] = dEps.v[c];
+        (a.eps + blk0 + c * nipt)[lo] = eps.v[c];
+        (a.epsplas + blk0 + c * nipt)[lo] = epl.v[c];
+        (a.deps + blk0 + c * nipt)[lo] = dEps.v[c];
       }
   }
 
@@ -421,7 +492,7 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long e_ra
   }
   if (valid)
   {
-    double *fo = a.fe + (long long)a.fpos[e * 8 + L] * 3;
+    double *fo = a.fe + (long long)(a.fpos + blk0)[lo] * 3;
     fo[0] = f0; fo[1] = f1; fo[2] = f2;
   }
 }

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kHexNT) void k_elem_hex8(ElemArgs a)
 {
   if ((a.check_running && !a.st->running) || a.st->halted) return;
   __shared__ HexLds<kHexNT> Ls;
-  hex_fill_table(a, Ls);
+  hex_fill_table(Ls);
   __syncthreads();
   // Workgroups are dispatched to the 8 XCDs round-robin, so neighbouring
   // elements, which share nodes, would each pull the nodes' lines into a
@@ -49,8 +49,7 @@ __global__ __launch_bounds__(kHexNT) void k_elem_hex8(ElemArgs a)
     const unsigned x = b & 7, k = b >> 3;
     lb = ((k / kRun) * 8 + x) * kRun + k % kRun;
   }
-  const long long e_raw = ((long long)lb * kHexNT + threadIdx.x) >> 3;
-  elem_hex8_body<SPEED_BC, TRACK, JC, kHexNT>(a, e_raw, a.ne, threadIdx.x, Ls);
+  elem_hex8_body<SPEED_BC, TRACK, JC, kHexNT>(a, (long long)lb * kHexNT, threadIdx.x, Ls);
 }
 
 template <bool SPEED_BC, bool TRACK, bool JC>

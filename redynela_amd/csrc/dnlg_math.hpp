@@ -366,19 +366,24 @@ DNL_HD bool inv3(const T9 &J, T9 &out)
 
 // one row of dN = dN0 * invJ in the dgemm NN order (LP/Blas/dgemm.c: for
 // each column j, C(i,j) = 0 then += (1*B(l,j)) * A(i,l) for l with
-// B(l,j) != 0).  The zero test is value-neutral and left out: the dN0 entries
-// are finite and non-zero, so a skipped term would add +-0 to a partial sum
-// that is never -0 (it starts at +0 and sums of non-zero terms that cancel
-// give +0), and 1*b is b.
+// B(l,j) != 0), starting from the first product instead of from +0 and
+// without the zero test.  Both are value-neutral.  The dN0 entries are
+// finite and non-zero, so a product is +-0 exactly when the dgemm skips its
+// term; dgemm's sum is therefore 0 + (its non-zero products in order), and
+// the sum here carries the same non-zero products in the same order with
+// +-0 terms in between: +-0 + p = p and s + (+-0) = s for s != 0, a
+// cancellation s + p = 0 gives +0 in both, and the two sums can only differ
+// if every product is +-0, i.e. if column j of invJ is zero, which an
+// inverse never has.  1*b is b.
 template <int D>
 DNL_HD void dgemm_row(const double *g, const double *inv, double *out)
 {
 #pragma unroll
   for (int j = 0; j < D; j++)
   {
-    double c = 0.;
+    double c = inv[j] * g[0];
 #pragma unroll
-    for (int l = 0; l < D; l++) c = c + inv[l * D + j] * g[l];
+    for (int l = 1; l < D; l++) c = c + inv[l * D + j] * g[l];
     out[j] = c;
   }
 }

@@ -157,3 +157,17 @@ def test_batch_create_rejects_bad_arguments_without_touching_the_gpu():
     assert L.dnlg_batch_set_engine(None, 0) == 1
     assert L.dnlg_batch_get_tuning(None, None, None) == 1
     assert L.dnlg_batch_solve(None, 1e-6, 10, None, None) == 1
+
+
+def test_hex8_compile_time_shape_constants_are_the_closed_forms():
+    """The FP64 hex8 kernel fills its shape-function table from constants
+    written as hexadecimal literals; they must be the doubles that the closed
+    forms of FE/ElHex8n3D.C:81-87 round to, which the host computes for the
+    other kernels (IEEE sqrt and division are correctly rounded in Python as in C)."""
+    import math
+    src = open(os.path.join(ROOT, "redynela_amd", "csrc", "dnlg_elem_f64.hpp")).read()
+    lit = dict(re.findall(r"constexpr double (kHexC[012]) = (0x[0-9a-fp.+-]+);", src))
+    assert sorted(lit) == ["kHexC0", "kHexC1", "kHexC2"]
+    assert float.fromhex(lit["kHexC0"]) == (2.0 - math.sqrt(3.0)) / 12.0
+    assert float.fromhex(lit["kHexC1"]) == 1.0 / 12.0
+    assert float.fromhex(lit["kHexC2"]) == (2.0 + math.sqrt(3.0)) / 12.0
