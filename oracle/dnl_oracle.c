@@ -104,9 +104,10 @@
  * the reference calls glibc log/pow.  Only IEEE +,-,*,/ are used, so any
  * IEEE-754 binary64 machine evaluating the same sequence produces the same
  * bits.  log: argument reduction x = 2^k (1+f), sqrt(1/2) <= 1+f < sqrt(2),
- * s = f/(2+f), log(1+f) = f - hfsq + s (hfsq + R(s^2)) with the Taylor
- * series of 2 atanh(s) (the arrangement of fdlibm's e_log.c, Taylor
- * coefficients 2/(2j+1) instead of minimax ones).  exp: Cody-Waite
+ * s = f/(2+f), log(1+f) = f - hfsq + s (hfsq + R(s^2)), R(z) = Lg1 z + ...
+ * + Lg7 z^7 the minimax polynomial and arrangement of fdlibm's e_log.c
+ * (|R - (2 atanh(s)/s - 2)| < 2^-58.45 for |s| <= 0.1716; its even and odd
+ * halves summed as there), the one final formula for every k.  exp: Cody-Waite
  * reduction by ln 2 and a degree-13 Taylor polynomial.  pow(x,y) for x>0:
  * exp(y log x), with y == 1 and y == 0 returned exactly.
  * ======================================================================== */
@@ -130,18 +131,11 @@ double dnlo_det_log(double x)
   double f = m - 1.0;
   double s = f / (2.0 + f);
   double z = s * s;
-  double R = z * (0.66666666666666666667 +
-             z * (0.40000000000000000000 +
-             z * (0.28571428571428571429 +
-             z * (0.22222222222222222222 +
-             z * (0.18181818181818181818 +
-             z * (0.15384615384615384615 +
-             z * (0.13333333333333333333 +
-             z * (0.11764705882352941176 +
-             z * (0.10526315789473684211 +
-             z * (0.09523809523809523810 +
-             z * (0.08695652173913043478 +
-             z * (0.08000000000000000000))))))))))));
+  double w = z * z;
+  double t1 = w * (3.999999999940941908e-01 + w * (2.222219843214978396e-01 + w * 1.531383769920937332e-01));
+  double t2 = z * (6.666666666666735130e-01 + w * (2.857142874366239149e-01 +
+              w * (1.818357216161805012e-01 + w * 1.479819860511658591e-01)));
+  double R = t2 + t1;
   double hfsq = 0.5 * f * f;
   double dk = (double)k;
   return dk * o_ln2_hi - ((hfsq - (s * (hfsq + R) + dk * o_ln2_lo)) - f);

@@ -257,13 +257,8 @@ struct QuadLds
   double x[15][NT + 1];   // rows 2k+d: dN(k, d); 8..12: Sig xx, xy, yx, yy, zz; 13: w detJ; 14: r
 };
 
-// fill the tables (all threads of the block; a barrier must follow); the
-// hex8 table is a copy of the compile-time one
-template <int NT>
-__device__ __forceinline__ void hex_fill_table(HexLds<NT> &L)
-{
-  for (int i = threadIdx.x; i < 8 * 8 * 3 + 4; i += NT) L.tab[i] = kHexTabDev.v[i];
-}
+// fill the quad4 tables (all threads of the block; a barrier must follow);
+// the hex8 kernel copies kHexTabDev instead
 template <int NT>
 __device__ __forceinline__ void quad_fill_table(const ElemArgs &a, QuadLds<NT> &L)
 {
@@ -289,6 +284,17 @@ __device__ __forceinline__ void wave_lds_sync()
 // ---------------------------------------------------------------------------
 // hex8 element kernel: lane = 8*element + integration point
 // ---------------------------------------------------------------------------
+// The lane offset of point tid of the workgroup whose first point is blk0: tid
+// itself, or for a lane past the end of the mesh the same point of the last
+// element (in bounds, never written).  The points of the workgroup that exist
+// are a multiple of 8, at least 8.
+template <int NT>
+__device__ __forceinline__ unsigned hex_lane_offset(const ElemArgs &a, long long blk0, int tid)
+{
+  const long long nvalid = a.ne * 8 - blk0;
+  return (tid < nvalid ? (unsigned)tid : (unsigned)(nvalid - 8) + (unsigned)(tid & 7)) & (unsigned)(NT - 1);
+}
+// ---------------------------------------------------------------------------
 // One integration point of one element per lane; lanes 8k..8k+7 of a wave
 // hold one element.  The workgroup's NT lanes take the points blk0 ..
 // blk0 + NT - 1 (blk0 a multiple of NT); tid indexes them and the exchange
@@ -300,28 +306,25 @@ __device__ __forceinline__ void wave_lds_sync()
 // shared by every access, instead of a 64-bit address per array and
 // component in vector registers.
 template <bool SPEED_BC, bool TRACK, bool JC, int NT>
-__device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long blk0, int tid, HexLds<NT> &Ls)
+__device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long blk0, int tid, unsigned lo, int mynode,
+                                               HexLds<NT> &Ls)
 {
   static_assert((NT & (NT - 1)) == 0 && NT % 64 == 0, "whole waves, a power of two");
   StepState *st = a.st;
   const int wl = tid & 63;
   const int pt = wl & 7, base = wl & ~7, lbase = tid & ~7;
   const long long nipt = a.ne * 8;
-  const long long nvalid = nipt - blk0;  // points of the workgroup that exist: a multiple of 8, at least 8
-  const bool valid = tid < nvalid;
-  // a lane past the end takes the same point of the last element (in bounds,
-  // never written)
-  const unsigned lo = (valid ? (unsigned)tid : (unsigned)(nvalid - 8) + (unsigned)pt) & (unsigned)(NT - 1);
+  const bool valid = tid < nipt - blk0;
   const long long e = (blk0 + lo) >> 3;
   // component c of S through a raw buffer view of the workgroup's points
   // (base a.S + c * nipt + blk0, in scalar registers), all nine at the one
   // lane offset: plain pointers would make the compiler fold blk0 + c * nipt
   // into a 64-bit address per lane and component, held from load to store
   const unsigned lob = lo * 8u;
-  // lane pt of the element fetches local node pt (coordinates and Current
-  // increment) and the element's 8 lanes share them through LDS rows 9..14,
-  // which hold Sig and w detJ only later in the pass
-  const int mynode = (a.conn + blk0)[lo];
+  // lane pt of the element has fetched local node pt (mynode, the caller's
+  // connectivity load); it loads the node's coordinates and Current increment
+  // and the element's 8 lanes share them through LDS rows 9..14, which hold
+  // Sig and w detJ only later in the pass
   {
     const double *xk = a.x + 3 * (long long)mynode;
     const double *uk = a.du + 3 * (long long)mynode;

@@ -29,10 +29,6 @@ constexpr int kHexNT = 256;
 template <bool SPEED_BC, bool TRACK, bool JC>
 __global__ __launch_bounds__(kHexNT) void k_elem_hex8(ElemArgs a)
 {
-  if ((a.check_running && !a.st->running) || a.st->halted) return;
-  __shared__ HexLds<kHexNT> Ls;
-  hex_fill_table(Ls);
-  __syncthreads();
   // Workgroups are dispatched to the 8 XCDs round-robin, so neighbouring
   // elements, which share nodes, would each pull the nodes' lines into a
   // different XCD's L2.  Runs of 8 workgroups (256 elements) go to one XCD
@@ -49,7 +45,19 @@ __global__ __launch_bounds__(kHexNT) void k_elem_hex8(ElemArgs a)
     const unsigned x = b & 7, k = b >> 3;
     lb = ((k / kRun) * 8 + x) * kRun + k % kRun;
   }
-  elem_hex8_body<SPEED_BC, TRACK, JC, kHexNT>(a, (long long)lb * kHexNT, threadIdx.x, Ls);
+  const long long blk0 = (long long)lb * kHexNT;
+  // the connectivity and table loads go out before the step-state check and
+  // the barrier, so that the three latencies overlap instead of adding up
+  const unsigned lo = hex_lane_offset<kHexNT>(a, blk0, threadIdx.x);
+  const int mynode = (a.conn + blk0)[lo];
+  constexpr int kTab = 8 * 8 * 3 + 4;
+  const double tab = threadIdx.x < kTab ? kHexTabDev.v[threadIdx.x] : 0.0;
+  const int running = a.st->running, halted = a.st->halted;  // one wait for both
+  if ((a.check_running && !running) || halted) return;
+  __shared__ HexLds<kHexNT> Ls;
+  if (threadIdx.x < kTab) Ls.tab[threadIdx.x] = tab;
+  __syncthreads();
+  elem_hex8_body<SPEED_BC, TRACK, JC, kHexNT>(a, blk0, threadIdx.x, lo, mynode, Ls);
 }
 
 template <bool SPEED_BC, bool TRACK, bool JC>

@@ -29,8 +29,9 @@ template <typename T> DNL_HD T dabs(T x) { return x < 0 ? -x : x; }  // Abs, BT/
 // ---------------------------------------------------------------------------
 // Deterministic log / exp / pow (no reference counterpart; the reference uses
 // glibc).  log: x = 2^k (1+f) with sqrt(1/2) <= 1+f < sqrt(2), s = f/(2+f),
-// log(1+f) = f - hfsq + s (hfsq + R(s^2)), R the Taylor series of
-// 2 atanh(s)/s - 2 truncated after s^24 (fdlibm e_log.c arrangement).
+// log(1+f) = f - hfsq + s (hfsq + R(s^2)), R(z) = Lg1 z + ... + Lg7 z^7 the
+// minimax polynomial of fdlibm's e_log.c (|R - (2 atanh(s)/s - 2)| <
+// 2^-58.45 on |s| <= 0.1716), summed as its even and odd halves.
 // exp: Cody-Waite reduction by ln 2, degree-13 Taylor polynomial.
 // ---------------------------------------------------------------------------
 DNL_HD double det_log(double x)
@@ -46,18 +47,11 @@ DNL_HD double det_log(double x)
   const double f = m - 1.0;
   const double s = f / (2.0 + f);
   const double z = s * s;
-  const double R = z * (0.66666666666666666667 +
-                   z * (0.40000000000000000000 +
-                   z * (0.28571428571428571429 +
-                   z * (0.22222222222222222222 +
-                   z * (0.18181818181818181818 +
-                   z * (0.15384615384615384615 +
-                   z * (0.13333333333333333333 +
-                   z * (0.11764705882352941176 +
-                   z * (0.10526315789473684211 +
-                   z * (0.09523809523809523810 +
-                   z * (0.08695652173913043478 +
-                   z * (0.08000000000000000000))))))))))));
+  const double w = z * z;
+  const double t1 = w * (3.999999999940941908e-01 + w * (2.222219843214978396e-01 + w * 1.531383769920937332e-01));
+  const double t2 = z * (6.666666666666735130e-01 + w * (2.857142874366239149e-01 +
+                    w * (1.818357216161805012e-01 + w * 1.479819860511658591e-01)));
+  const double R = t2 + t1;
   const double hfsq = 0.5 * f * f;
   const double dk = (double)k;
   return dk * 6.93147180369123816490e-01 - ((hfsq - (s * (hfsq + R) + dk * 1.90821492927058770002e-10)) - f);
