@@ -63,13 +63,33 @@ __global__ __launch_bounds__(64) void k_bstep_dt(BatchArgs B, int nb, int preamb
 // [nn][3] rows of the block's nodes through LDS, so that global loads and
 // stores cover whole contiguous spans (the block's nodes are contiguous and
 // end at the bar's last node at most)
+// (the loads of all four fields go out first, at indices clamped into the
+// bar's rows, and the LDS writes follow: see RowLoad)
 template <int NT>
-__device__ __forceinline__ void bstage_in(const double *g, double *l, long long i0, long long iend)
+struct BRowLoad
 {
-  const long long base = 3 * i0, lim = 3 * iend;
+  double v[3];
+  __device__ __forceinline__ void load(const double *g, long long i0, long long iend)
+  {
+    const long long base = 3 * i0, lim = 3 * iend;
 #pragma unroll
-  for (int j = threadIdx.x; j < 3 * NT; j += NT) l[j] = (base + j < lim) ? g[base + j] : 0.0;
-}
+    for (int k = 0; k < 3; k++)
+    {
+      const long long j = base + threadIdx.x + k * NT;
+      v[k] = g[j < lim ? j : lim - 1];
+    }
+  }
+  __device__ __forceinline__ void store(double *l, long long i0, long long iend) const
+  {
+    const long long base = 3 * i0, lim = 3 * iend;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+    {
+      const int j = threadIdx.x + k * NT;
+      l[j] = base + j < lim ? v[k] : 0.0;
+    }
+  }
+};
 template <int NT>
 __device__ __forceinline__ void bstage_out(double *g, const double *l, long long i0, long long iend)
 {
@@ -93,10 +113,17 @@ __global__ __launch_bounds__(NT) void k_bstep_nodal(BatchArgs B, const int2 *blk
   const NodalArgs &n = B.na;
   const long long i0 = bb.y, iend = d.node0 + d.nn, i = i0 + threadIdx.x;
   const int t3 = 3 * threadIdx.x;
-  bstage_in<NT>(n.x, sx, i0, iend);
-  bstage_in<NT>(n.v, sv, i0, iend);
-  bstage_in<NT>(n.a, sa, i0, iend);
-  bstage_in<NT>(n.d, sd, i0, iend);
+  {
+    BRowLoad<NT> rx, rv, ra, rd;
+    rx.load(n.x, i0, iend);
+    rv.load(n.v, i0, iend);
+    ra.load(n.a, i0, iend);
+    rd.load(n.d, i0, iend);
+    rx.store(sx, i0, iend);
+    rv.store(sv, i0, iend);
+    ra.store(sa, i0, iend);
+    rd.store(sd, i0, iend);
+  }
   __syncthreads();
   double dun[3] = {0.0, 0.0, 0.0};
   if (i < iend)
@@ -128,16 +155,21 @@ __global__ __launch_bounds__(NT) void k_bstep_nodal(BatchArgs B, const int2 *blk
 
 // FP64 element pass: NT / 4 elements (NT points) per block; 5 waves per SIMD
 // (96 VGPRs, no spills; the LDS of either tile allows as many)
+// The block table, the bar descriptors and the per-bar arguments come in as
+// restrict-qualified read-only pointers of their own, so that the compiler
+// may read them with scalar loads at the start; through BatchArgs every
+// argument field is a vector load waited on where it is used.
 template <int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_bstep_elem(BatchArgs B, const int2 *blk,
-                                                                                          int preamble)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_bstep_elem(
+    BatchArgs B, const int2 *__restrict__ blk, const BarDesc *__restrict__ bars, const ElemArgs *__restrict__ bea,
+    int preamble)
 {
   __shared__ QuadLds<NT> Ls;
   const int2 bb = blk[blockIdx.x];
   StepState *st = B.states + bb.x;
   if (!st->running || st->status) return;
-  const BarDesc &d = B.bars[bb.x];
-  const ElemArgs &a = B.bar_ea[bb.x];
+  const BarDesc &d = bars[bb.x];
+  const ElemArgs &a = bea[bb.x];
   quad_fill_table(a, Ls);
   __syncthreads();
   const long long e_end = d.elem0 + d.ne, e_raw = (long long)bb.y + (threadIdx.x >> 2);
@@ -145,7 +177,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_
   if (preamble || (st->increment % st->freq) == 0)
   {
     double l = __builtin_inf();
-    if ((threadIdx.x & 3) == 0 && e_raw < e_end) l = quad_length(a.x, a.conn + e_raw * 4) / st->cs;
+    if ((threadIdx.x & 3) == 0 && e_raw < e_end) l = quad_length(gmem(a.x), gmem(a.conn) + e_raw * 4) / st->cs;
     courant_contrib(l, B.crit + bb.x);
   }
 }
@@ -159,8 +191,8 @@ void launch_batch_f64(const BatchArgs &B, int nbars, hipStream_t s)
 // tile 0: 64-lane blocks; tile 1: 256-lane blocks (the block tables match)
 void launch_bstep_elem_f64(const BatchArgs &B, const int2 *blk, int nblk, int preamble, int tile, hipStream_t s)
 {
-  if (tile) hipLaunchKernelGGL(k_bstep_elem<256>, dim3(nblk), dim3(256), 0, s, B, blk, preamble);
-  else hipLaunchKernelGGL(k_bstep_elem<64>, dim3(nblk), dim3(64), 0, s, B, blk, preamble);
+  if (tile) hipLaunchKernelGGL(k_bstep_elem<256>, dim3(nblk), dim3(256), 0, s, B, blk, B.bars, B.bar_ea, preamble);
+  else hipLaunchKernelGGL(k_bstep_elem<64>, dim3(nblk), dim3(64), 0, s, B, blk, B.bars, B.bar_ea, preamble);
 }
 
 void launch_bstep_dt(const BatchArgs &B, int nb, int preamble, hipStream_t s)

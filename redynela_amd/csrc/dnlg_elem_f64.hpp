@@ -89,6 +89,16 @@ __device__ __forceinline__ void buf_st(double *p, unsigned off, double v)
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(dnl_u2, v), r, off, 0, 0);
 }
 
+// p as a pointer into global memory.  Every array the kernels touch is device
+// global memory, but a pointer read from memory (the per-bar arguments of the
+// batched kernels) reaches the compiler as a generic one, and the flat loads
+// and stores it then emits also count against the LDS wait counter.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p)
+{
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 __device__ __forceinline__ void raise_status(StepState *st, int code, long long e)
 {
   if (atomicCAS(&st->status, 0, code) == 0) st->err_elem = (int)e;
@@ -513,7 +523,7 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
   const bool valid = e_raw < e_end;
   const long long e = valid ? e_raw : e_end - 1;
   const long long nipt = a.ne * 4, ip = e * 4 + pt;
-  const int *cn = a.conn + e * 4;
+  const auto *cn = gmem(a.conn) + e * 4;
   int nd[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) nd[k] = cn[k];
@@ -523,7 +533,7 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
 #pragma unroll
   for (int k = 0; k < 4; k++)
   {
-    const double *xk = a.x + 3 * (long long)nd[k];
+    const auto *xk = gmem(a.x) + 3 * (long long)nd[k];
     const double x0 = xk[0], x1 = xk[1];
     const double *tg = Ls.tab + (pt * 4 + k) * 3;
     const double g[2] = {tg[0], tg[1]};
@@ -532,7 +542,7 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
   }
   // getRadius_atIntPoint, FE/Element2D.C:233-246
 #pragma unroll
-  for (int k = 0; k < 4; k++) Rr += Ls.tab[(pt * 4 + k) * 3 + 2] * a.x[3 * (long long)nd[k]];
+  for (int k = 0; k < 4; k++) Rr += Ls.tab[(pt * 4 + k) * 3 + 2] * gmem(a.x)[3 * (long long)nd[k]];
   const double detJ = (J0 * J3) - (J1 * J2);
   if (detJ < 0. && valid) raise_status(st, ST_DETJ, e);
   const double inv[4] = {J3 / detJ, -J1 / detJ, -J2 / detJ, J0 / detJ};
@@ -543,14 +553,14 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
 #pragma unroll
   for (int k = 0; k < 4; k++)
   {
-    const double *uk = a.du + 3 * (long long)nd[k];
+    const auto *uk = gmem(a.du) + 3 * (long long)nd[k];
     double u0 = uk[0], u1 = uk[1];
     if (SPEED_BC)
     {
-      const unsigned char m = a.bcs[nd[k]];
+      const unsigned char m = gmem(a.bcs)[nd[k]];
       if (m)
       {
-        const double *sp = a.bcs_v + 3 * (long long)nd[k];
+        const auto *sp = gmem(a.bcs_v) + 3 * (long long)nd[k];
         if (m & 1) u0 = sp[0] * st->dt;
         if (m & 2) u1 = sp[1] * st->dt;
       }
@@ -571,20 +581,20 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
   if (!polar_lnU(F, dEps, R) && valid) raise_status(st, ST_POLAR, e);
   T9 S;
 #pragma unroll
-  for (int c = 0; c < 9; c++) S.v[c] = a.S[c * nipt + ip];
-  double evp = a.evp[ip];
+  for (int c = 0; c < 9; c++) S.v[c] = gmem(a.S)[c * nipt + ip];
+  double evp = gmem(a.evp)[ip];
   T9 eps, epl;
   if (TRACK)
   {
 #pragma unroll
-    for (int c = 0; c < 9; c++) { eps.v[c] = a.eps[c * nipt + ip] + dEps.v[c]; epl.v[c] = a.epsplas[c * nipt + ip]; }
+    for (int c = 0; c < 9; c++) { eps.v[c] = gmem(a.eps)[c * nipt + ip] + dEps.v[c]; epl.v[c] = gmem(a.epsplas)[c * nipt + ip]; }
   }
   const double tr = dEps.v[0] + dEps.v[4] + dEps.v[8];
   double Tip = 0.0;
   bool rr_ok;
   if (JC)  // law 2, a separate instance: its registers do not weigh on law 1
   {
-    Tip = a.T[ip];
+    Tip = gmem(a.T)[ip];
     rr_ok = stress_update_jc(a, dEps, S, evp, Tip, st->dt, TRACK ? &epl : nullptr);
   }
   else
@@ -594,7 +604,7 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
 #pragma unroll
   for (int q = 0; q < 4; q++) pressure += __shfl(tr, base + q, 64);
   pressure /= 4;
-  const double p = a.p[e] + a.K * pressure;
+  const double p = gmem(a.p)[e] + a.K * pressure;
   S = mul(R, mul_bt(S, R));
   if (TRACK) { eps = mul(R, mul_bt(eps, R)); epl = mul(R, mul_bt(epl, R)); }
   T9 Sig;
@@ -606,17 +616,17 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
   if (valid)
   {
 #pragma unroll
-    for (int c = 0; c < 9; c++) a.S[c * nipt + ip] = S.v[c];
-    a.evp[ip] = evp;
-    if (JC) a.T[ip] = Tip;
-    if (pt == 0) a.p[e] = p;
+    for (int c = 0; c < 9; c++) gmem(a.S)[c * nipt + ip] = S.v[c];
+    gmem(a.evp)[ip] = evp;
+    if (JC) gmem(a.T)[ip] = Tip;
+    if (pt == 0) gmem(a.p)[e] = p;
     if (TRACK)
 #pragma unroll
       for (int c = 0; c < 9; c++)
       {
-        a.eps[c * nipt + ip] = eps.v[c];
-        a.epsplas[c * nipt + ip] = epl.v[c];
-        a.deps[c * nipt + ip] = dEps.v[c];
+        gmem(a.eps)[c * nipt + ip] = eps.v[c];
+        gmem(a.epsplas)[c * nipt + ip] = epl.v[c];
+        gmem(a.deps)[c * nipt + ip] = dEps.v[c];
       }
   }
 
@@ -642,7 +652,7 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
   }
   if (valid)
   {
-    double *fo = a.fe + (long long)a.fpos[e * 4 + L] * 2;
+    auto *fo = gmem(a.fe) + (long long)gmem(a.fpos)[e * 4 + L] * 2;
     fo[0] = f0; fo[1] = f1;
   }
 }
@@ -767,11 +777,12 @@ __device__ __forceinline__ void nodal_core(const NodalArgs &n, long long i, cons
 }
 
 // ElQua4nAx::getLength, FE/ElQua4nAx.C:131-143
-__device__ __forceinline__ double quad_length(const double *x, const int *cn)
+template <class XP, class CP>
+__device__ __forceinline__ double quad_length(XP x, CP cn)
 {
 #pragma clang fp contract(off)
-  const double *x0 = x + 3 * (long long)cn[0], *x1 = x + 3 * (long long)cn[1], *x2 = x + 3 * (long long)cn[2],
-               *x3 = x + 3 * (long long)cn[3];
+  const auto x0 = x + 3 * (long long)cn[0], x1 = x + 3 * (long long)cn[1], x2 = x + 3 * (long long)cn[2],
+             x3 = x + 3 * (long long)cn[3];
   const double x31 = x2[0] - x0[0];
   const double x24 = x1[0] - x3[0];
   const double y42 = x3[1] - x1[1];

@@ -175,13 +175,36 @@ __global__ __launch_bounds__(256) void k_gather_fint(const unsigned char *deg, c
 // ---------------------------------------------------------------------------
 
 // [nn][3] AoS arrays are moved through LDS so that every global load and
-// store of a block covers whole contiguous lines (3 x 256 doubles per field)
-__device__ __forceinline__ void stage_in(const double *g, double *l, long long i0, long long nn)
+// store of a block covers whole contiguous lines (3 x 256 doubles per field).
+// Each lane loads its three values of every field first, at indices clamped
+// into the array so that the loads carry no branch and go out back to back,
+// and writes them to LDS afterwards (zeros past the end): a conditional load
+// per value would wait on each load before issuing the next.
+template <int NT>
+struct RowLoad
 {
-  const long long base = 3 * i0, lim = 3 * nn;
+  double v[3];
+  __device__ __forceinline__ void load(const double *g, long long i0, long long nn)
+  {
+    const long long base = 3 * i0, lim = 3 * nn;
 #pragma unroll
-  for (int j = threadIdx.x; j < 768; j += 256) l[j] = (base + j < lim) ? g[base + j] : 0.0;
-}
+    for (int k = 0; k < 3; k++)
+    {
+      const long long j = base + threadIdx.x + k * NT;
+      v[k] = g[j < lim ? j : lim - 1];
+    }
+  }
+  __device__ __forceinline__ void store(double *l, long long i0, long long nn) const
+  {
+    const long long base = 3 * i0, lim = 3 * nn;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+    {
+      const int j = threadIdx.x + k * NT;
+      l[j] = base + j < lim ? v[k] : 0.0;
+    }
+  }
+};
 __device__ __forceinline__ void stage_out(double *g, const double *l, long long i0, long long nn)
 {
   const long long base = 3 * i0, lim = 3 * nn;
@@ -202,10 +225,17 @@ __global__ __launch_bounds__(256) void k_gather_nodal(NodalArgs n)
   const long long i0 = (long long)blockIdx.x * 256;
   const long long i = i0 + threadIdx.x;
   const int t3 = 3 * threadIdx.x;
-  stage_in(n.x, sx, i0, n.nn);
-  stage_in(n.v, sv, i0, n.nn);
-  stage_in(n.a, sa, i0, n.nn);
-  if (TRACK_D) stage_in(n.d, sd, i0, n.nn);
+  {
+    RowLoad<256> rx, rv, ra, rd;
+    rx.load(n.x, i0, n.nn);
+    rv.load(n.v, i0, n.nn);
+    ra.load(n.a, i0, n.nn);
+    if (TRACK_D) rd.load(n.d, i0, n.nn);
+    rx.store(sx, i0, n.nn);
+    rv.store(sv, i0, n.nn);
+    ra.store(sa, i0, n.nn);
+    if (TRACK_D) rd.store(sd, i0, n.nn);
+  }
   __syncthreads();
   double ke = 0.0, dun[3] = {0.0, 0.0, 0.0};
   if (i < n.nn)
