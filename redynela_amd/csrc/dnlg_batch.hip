@@ -170,7 +170,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
   if (!st->running || st->status) return;
   const BarDesc &d = bars[bb.x];
   const ElemArgs &a = bea[bb.x];
-  quad_fill_table(a, Ls);
+  quad_fill_table(Ls);
   __syncthreads();
   const long long e_end = d.elem0 + d.ne, e_raw = (long long)bb.y + (threadIdx.x >> 2);
   elem_quad4ax_body<false, false, false, NT>(a, e_raw, e_end, threadIdx.x, Ls);
@@ -185,7 +185,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
 void launch_batch_f64(const BatchArgs &B, int nbars, hipStream_t s)
 {
-  hipLaunchKernelGGL((k_batch_quad4<ElemPassF64, false>), dim3(nbars), dim3(256), 0, s, B);
+  hipLaunchKernelGGL((k_batch_quad4<ElemPassF64, false>), dim3(nbars), dim3(256), 0, s, B, B.order, B.bars, B.bar_ea,
+                     B.bar_ea32);
 }
 
 // tile 0: 64-lane blocks; tile 1: 256-lane blocks (the block tables match)

@@ -15,7 +15,7 @@ namespace dnl {
 __device__ __forceinline__ double bar_courant(const ElemArgs &a, long long e0, long long e1, double *red)
 {
   double t = __builtin_inf();
-  for (long long e = e0 + threadIdx.x; e < e1; e += 256) t = fmin(t, quad_length(a.x, a.conn + e * 4) / (a.st->cs));
+  for (long long e = e0 + threadIdx.x; e < e1; e += 256) t = fmin(t, quad_length(gmem(a.x), gmem(a.conn) + e * 4) / (a.st->cs));
   red[threadIdx.x] = t;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1)
@@ -80,19 +80,25 @@ struct ElemPassF64
   }
 };
 
+// The bar order, descriptors and per-bar arguments come in as restrict-qualified
+// read-only pointers of their own so that the compiler reads them with scalar
+// loads (see k_bstep_elem).
 template <class PASS, bool FE32>
-__global__ __launch_bounds__(256) void k_batch_quad4(BatchArgs B)
+__global__ __launch_bounds__(256) void k_batch_quad4(BatchArgs B, const int *__restrict__ order,
+                                                     const BarDesc *__restrict__ bars,
+                                                     const ElemArgs *__restrict__ bea,
+                                                     const ElemArgs32 *__restrict__ bea32)
 {
   __shared__ QuadLds<256> Ls;
   __shared__ double red[256];
   __shared__ int s_go, s_rec;
-  const int bar = B.order[blockIdx.x];
-  const BarDesc d = B.bars[bar];
+  const int bar = order[blockIdx.x];
+  const BarDesc d = bars[bar];
   StepState *st = B.states + bar;
-  const ElemArgs &a = B.bar_ea[bar];      // in global memory: no per-lane copy
-  const ElemArgs32 &a32 = B.bar_ea32[bar];
+  const ElemArgs &a = bea[bar];      // in global memory: no per-lane copy
+  const ElemArgs32 &a32 = bea32[bar];
   const long long e0 = d.elem0, e1 = d.elem0 + d.ne, n0 = d.node0, n1 = d.node0 + d.nn;
-  quad_fill_table(a, Ls);
+  quad_fill_table(Ls);
   __syncthreads();
   if (!st->running)
   {

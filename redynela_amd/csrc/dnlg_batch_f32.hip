@@ -19,18 +19,21 @@ struct ElemPassF32
 
 void launch_batch_f32(const BatchArgs &B, int nbars, hipStream_t s)
 {
-  hipLaunchKernelGGL((k_batch_quad4<ElemPassF32, true>), dim3(nbars), dim3(256), 0, s, B);
+  hipLaunchKernelGGL((k_batch_quad4<ElemPassF32, true>), dim3(nbars), dim3(256), 0, s, B, B.order, B.bars, B.bar_ea,
+                     B.bar_ea32);
 }
 
 // stepped engine, mixed form: one element per lane, 64 elements per block
 template <int NT>
-__global__ __launch_bounds__(NT) void k_bstep_elem32(BatchArgs B, const int2 *blk, int preamble)
+__global__ __launch_bounds__(NT) void k_bstep_elem32(BatchArgs B, const int2 *__restrict__ blk,
+                                                    const BarDesc *__restrict__ bars,
+                                                    const ElemArgs32 *__restrict__ bea32, int preamble)
 {
   const int2 bb = blk[blockIdx.x];
   StepState *st = B.states + bb.x;
   if (!st->running || st->status) return;
-  const BarDesc &d = B.bars[bb.x];
-  const ElemArgs32 &a = B.bar_ea32[bb.x];
+  const BarDesc &d = bars[bb.x];
+  const ElemArgs32 &a = bea32[bb.x];
   const long long e_end = d.elem0 + d.ne, e = (long long)bb.y + threadIdx.x;
   if (e < e_end) elem_quad4ax_f32_body(a, e);
   if (preamble || (st->increment % st->freq) == 0)
@@ -44,8 +47,8 @@ __global__ __launch_bounds__(NT) void k_bstep_elem32(BatchArgs B, const int2 *bl
 // tile 0: 64 elements per block; tile 1: 256 (the block tables match)
 void launch_bstep_elem_f32(const BatchArgs &B, const int2 *blk, int nblk, int preamble, int tile, hipStream_t s)
 {
-  if (tile) hipLaunchKernelGGL(k_bstep_elem32<256>, dim3(nblk), dim3(256), 0, s, B, blk, preamble);
-  else hipLaunchKernelGGL(k_bstep_elem32<64>, dim3(nblk), dim3(64), 0, s, B, blk, preamble);
+  if (tile) hipLaunchKernelGGL(k_bstep_elem32<256>, dim3(nblk), dim3(256), 0, s, B, blk, B.bars, B.bar_ea32, preamble);
+  else hipLaunchKernelGGL(k_bstep_elem32<64>, dim3(nblk), dim3(64), 0, s, B, blk, B.bars, B.bar_ea32, preamble);
 }
 
 }  // namespace dnl

@@ -450,18 +450,18 @@ __device__ __forceinline__ void elem_quad4ax_f32_body(const ElemArgs32 &a, long 
   const float dt_inv = (a.law == 2) ? (float)(1.0 / st->dt) : 0.f;
   int nd[4];
 #pragma unroll
-  for (int k = 0; k < 4; k++) nd[k] = a.conn[e * 4 + k];
+  for (int k = 0; k < 4; k++) nd[k] = gmem(a.conn)[e * 4 + k];
   float xr[4][2], r[4], u[4][2];
-  const double x00 = a.x[3 * (long long)nd[0]], x01 = a.x[3 * (long long)nd[0] + 1];
+  const double x00 = gmem(a.x)[3 * (long long)nd[0]], x01 = gmem(a.x)[3 * (long long)nd[0] + 1];
 #pragma unroll
   for (int k = 0; k < 4; k++)
   {
-    const double *xk = a.x + 3 * (long long)nd[k];
+    const auto *xk = gmem(a.x) + 3 * (long long)nd[k];
     xr[k][0] = (float)(xk[0] - x00);
     xr[k][1] = (float)(xk[1] - x01);
     r[k] = (float)xk[0];
-    u[k][0] = (float)a.du[3 * (long long)nd[k]];
-    u[k][1] = (float)a.du[3 * (long long)nd[k] + 1];
+    u[k][0] = (float)gmem(a.du)[3 * (long long)nd[k]];
+    u[k][1] = (float)gmem(a.du)[3 * (long long)nd[k] + 1];
   }
   float Af[4][2], g[4][2];
 #pragma unroll
@@ -503,16 +503,16 @@ __device__ __forceinline__ void elem_quad4ax_f32_body(const ElemArgs32 &a, long 
     const long long nipt = 4 * ne;
     float S[6];
 #pragma unroll
-    for (int c = 0; c < 6; c++) S[c] = a.S[c * nipt + ip];
-    float evp = a.evp[ip];
-    float T = a.T ? a.T[ip] : 0.f;
+    for (int c = 0; c < 6; c++) S[c] = gmem(a.S)[c * nipt + ip];
+    float evp = gmem(a.evp)[ip];
+    float T = a.T ? gmem(a.T)[ip] : 0.f;
     ok = stress_f32(a, dE, S, evp, T, dt_inv) && ok;
     pressure += dE[0] + dE[1] + dE[2];
     rot_sym(R, S);
 #pragma unroll
-    for (int c = 0; c < 6; c++) a.S[c * nipt + ip] = S[c];
-    a.evp[ip] = evp;
-    if (a.T) a.T[ip] = T;
+    for (int c = 0; c < 6; c++) gmem(a.S)[c * nipt + ip] = S[c];
+    gmem(a.evp)[ip] = evp;
+    if (a.T) gmem(a.T)[ip] = T;
     const float W = detJ * twopi * Rr;
 #pragma unroll
     for (int k = 0; k < 4; k++)
@@ -529,12 +529,12 @@ __device__ __forceinline__ void elem_quad4ax_f32_body(const ElemArgs32 &a, long 
     }
   }
   if (!ok) raise32(st, ST_RR, e);
-  const float p = a.p[e] + a.K * (pressure * 0.25f);
-  a.p[e] = p;
+  const float p = gmem(a.p)[e] + a.K * (pressure * 0.25f);
+  gmem(a.p)[e] = p;
 #pragma unroll
   for (int k = 0; k < 4; k++)
   {
-    float *fo = a.fe + (long long)a.fpos[e * 4 + k] * 2;
+    auto *fo = gmem(a.fe) + (long long)gmem(a.fpos)[e * 4 + k] * 2;
     fo[0] = -(Af[k][0] + p * g[k][0]);
     fo[1] = -(Af[k][1] + p * g[k][1]);
   }

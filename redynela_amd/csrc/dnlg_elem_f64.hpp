@@ -59,6 +59,33 @@ constexpr HexTab make_hex_tab()
 }
 static __constant__ HexTab kHexTabDev = make_hex_tab();
 
+// The quad4 table of QuadLds::tab (dN0(ip, k, 0..1), N(ip, k)) built the same
+// way: (3 -+ sqrt 3)/12 and (2 -+ sqrt 3)/6, 1/6 of FE/ElQua4nAx.C:51-73,
+// correctly rounded -- the doubles of ElemArgs::c0, c1 and n0..n2 -- with
+// the signs of quad_dn0 and the choice of quad_n.
+constexpr double kQuadC0 = 0x1.b0cb174df99c8p-4;
+constexpr double kQuadC1 = 0x1.93cd3a2c8198dp-2;
+constexpr double kQuadN0 = 0x1.6dd707e2911cbp-5;
+constexpr double kQuadN1 = 0x1.5555555555555p-3;
+constexpr double kQuadN2 = 0x1.3e77e4d72c439p-1;
+struct QuadTab { double v[4 * 4 * 3]; };
+constexpr QuadTab make_quad_tab()
+{
+  QuadTab t{};
+  for (int ip = 0; ip < 4; ip++)
+    for (int k = 0; k < 4; k++)
+    {
+      const bool s0 = hex_csg(ip, 0) == hex_csg(k, 0), s1 = hex_csg(ip, 1) == hex_csg(k, 1);
+      const double m0 = s1 ? kQuadC1 : kQuadC0, m1 = s0 ? kQuadC1 : kQuadC0;
+      t.v[(ip * 4 + k) * 3 + 0] = hex_csg(k, 0) > 0 ? m0 : -m0;
+      t.v[(ip * 4 + k) * 3 + 1] = hex_csg(k, 1) > 0 ? m1 : -m1;
+      const int ag = s0 + s1;
+      t.v[(ip * 4 + k) * 3 + 2] = ag == 2 ? kQuadN2 : (ag == 1 ? kQuadN1 : kQuadN0);
+    }
+  return t;
+}
+static __constant__ QuadTab kQuadTabDev = make_quad_tab();
+
 // quad4 dN0(ip, node, d) = sign(node,d) * (3 +- r3)/12 (FE/ElQua4nAx.C:51-73)
 __device__ __forceinline__ void quad_dn0(const ElemArgs &a, int ip, int k, double *g)
 {
@@ -87,16 +114,6 @@ __device__ __forceinline__ void buf_st(double *p, unsigned off, double v)
 {
   const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(p, 0, -1, 0x00020000);
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(dnl_u2, v), r, off, 0, 0);
-}
-
-// p as a pointer into global memory.  Every array the kernels touch is device
-// global memory, but a pointer read from memory (the per-bar arguments of the
-// batched kernels) reaches the compiler as a generic one, and the flat loads
-// and stores it then emits also count against the LDS wait counter.
-template <class T>
-__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p)
-{
-  return (__attribute__((address_space(1))) T *)p;
 }
 
 __device__ __forceinline__ void raise_status(StepState *st, int code, long long e)
@@ -267,18 +284,12 @@ struct QuadLds
   double x[15][NT + 1];   // rows 2k+d: dN(k, d); 8..12: Sig xx, xy, yx, yy, zz; 13: w detJ; 14: r
 };
 
-// fill the quad4 tables (all threads of the block; a barrier must follow);
-// the hex8 kernel copies kHexTabDev instead
+// fill the quad4 table from the compile-time copy (all threads of the block;
+// a barrier must follow); the hex8 kernel copies kHexTabDev itself
 template <int NT>
-__device__ __forceinline__ void quad_fill_table(const ElemArgs &a, QuadLds<NT> &L)
+__device__ __forceinline__ void quad_fill_table(QuadLds<NT> &L)
 {
-  for (int i = threadIdx.x; i < 48; i += NT)
-  {
-    const int ip = i / 12, k = (i / 3) % 4, c = i % 3;
-    double g[2];
-    quad_dn0(a, ip, k, g);
-    L.tab[i] = c < 2 ? g[c] : quad_n(a, ip, k);
-  }
+  for (int i = threadIdx.x; i < 4 * 4 * 3; i += NT) L.tab[i] = kQuadTabDev.v[i];
 }
 
 // LDS exchange among the lanes of one wave: the writes of every lane before
@@ -528,8 +539,12 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
 #pragma unroll
   for (int k = 0; k < 4; k++) nd[k] = cn[k];
 
-  // FE/Element2D.C:84-115
-  double J0 = 0, J1 = 0, J2 = 0, J3 = 0, Rr = 0.;
+  // FE/Element2D.C:84-115 and getRadius_atIntPoint, FE/Element2D.C:233-246.
+  // The reference's sums start at +0; these start at node 0's product, the
+  // same double unless all four products of a sum are -0, which needs all four
+  // nodes at a zero coordinate: a flat element (J singular either way) or an
+  // element lying on the axis (radius).
+  double J0, J1, J2, J3, Rr;
 #pragma unroll
   for (int k = 0; k < 4; k++)
   {
@@ -537,12 +552,18 @@ __device__ __forceinline__ void elem_quad4ax_body(const ElemArgs &a, long long e
     const double x0 = xk[0], x1 = xk[1];
     const double *tg = Ls.tab + (pt * 4 + k) * 3;
     const double g[2] = {tg[0], tg[1]};
+    if (k == 0)
+    {
+      J0 = g[0] * x0; J1 = g[1] * x0;
+      J2 = g[0] * x1; J3 = g[1] * x1;
+      Rr = tg[2] * x0;
+      continue;
+    }
     J0 += g[0] * x0; J1 += g[1] * x0;
     J2 += g[0] * x1; J3 += g[1] * x1;
   }
-  // getRadius_atIntPoint, FE/Element2D.C:233-246
 #pragma unroll
-  for (int k = 0; k < 4; k++) Rr += Ls.tab[(pt * 4 + k) * 3 + 2] * gmem(a.x)[3 * (long long)nd[k]];
+  for (int k = 1; k < 4; k++) Rr += Ls.tab[(pt * 4 + k) * 3 + 2] * gmem(a.x)[3 * (long long)nd[k]];
   const double detJ = (J0 * J3) - (J1 * J2);
   if (detJ < 0. && valid) raise_status(st, ST_DETJ, e);
   const double inv[4] = {J3 / detJ, -J1 / detJ, -J2 / detJ, J0 / detJ};

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(64) void k_elem_quad4ax(ElemArgs a)
 {
   if ((a.check_running && !a.st->running) || a.st->halted) return;
   __shared__ QuadLds<64> Ls;
-  quad_fill_table(a, Ls);
+  quad_fill_table(Ls);
   __syncthreads();
   const long long e_raw = ((long long)blockIdx.x * 64 + threadIdx.x) >> 2;
   elem_quad4ax_body<SPEED_BC, TRACK, JC, 64>(a, e_raw, a.ne, threadIdx.x, Ls);

@@ -6,6 +6,16 @@
 
 namespace dnl {
 
+// p as a pointer into global memory.  Every array the kernels touch is device
+// global memory, but a pointer read from memory (the per-bar arguments of the
+// batched kernels) reaches the compiler as a generic one, and the flat loads
+// and stores it then emits also count against the LDS wait counter.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p)
+{
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 // device status codes (mirror include/dnlg.h)
 enum : int { ST_OK = 0, ST_POLAR = 2, ST_RR = 3, ST_DETJ = 4, ST_BUDGET = 6, ST_PULSE = 7 };
 

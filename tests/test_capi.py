@@ -171,3 +171,13 @@ def test_hex8_compile_time_shape_constants_are_the_closed_forms():
     assert float.fromhex(lit["kHexC0"]) == (2.0 - math.sqrt(3.0)) / 12.0
     assert float.fromhex(lit["kHexC1"]) == 1.0 / 12.0
     assert float.fromhex(lit["kHexC2"]) == (2.0 + math.sqrt(3.0)) / 12.0
+
+
+def test_quad4_compile_time_shape_constants_are_the_closed_forms():
+    """The same for the quad4 table (FE/ElQua4nAx.C:51-73): (3 -+ sqrt 3)/12 and (2 -+ sqrt 3)/6, 1/6."""
+    import math
+    src = open(os.path.join(ROOT, "redynela_amd", "csrc", "dnlg_elem_f64.hpp")).read()
+    lit = {k: float.fromhex(v) for k, v in re.findall(r"constexpr double (kQuad[CN][012]) = (0x[0-9a-fp.+-]+);", src)}
+    r3 = math.sqrt(3.0)
+    assert lit == {"kQuadC0": (3.0 - r3) / 12.0, "kQuadC1": (3.0 + r3) / 12.0, "kQuadN0": (2.0 - r3) / 6.0,
+                   "kQuadN1": 1.0 / 6.0, "kQuadN2": (2.0 + r3) / 6.0}
