@@ -426,16 +426,12 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long blk0
 #pragma unroll
   for (int c = 0; c < 9; c++) S.v[c] = buf_ld(a.S + c * nipt + blk0, lob);
   double evp = (a.evp + blk0)[lo];
+  // the strain diagnostics go through buffer views like S; the return needs
+  // only EpsPlas, and Eps (+ dEps) is read after it
   T9 eps, epl;
   if (TRACK)
-  {
 #pragma unroll
-    for (int c = 0; c < 9; c++)
-    {
-      eps.v[c] = (a.eps + blk0 + c * nipt)[lo] + dEps.v[c];
-      epl.v[c] = (a.epsplas + blk0 + c * nipt)[lo];
-    }
-  }
+    for (int c = 0; c < 9; c++) epl.v[c] = buf_ld(a.epsplas + c * nipt + blk0, lob);
   const double tr = dEps.v[0] + dEps.v[4] + dEps.v[8];
   double Tip = 0.0;
   bool rr_ok;
@@ -457,7 +453,13 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long blk0
 
   // RFI and Sig = S + p I, FE/Element.C:362-401
   S = mul(R, mul_bt(S, R));
-  if (TRACK) { eps = mul(R, mul_bt(eps, R)); epl = mul(R, mul_bt(epl, R)); }
+  if (TRACK)
+  {
+#pragma unroll
+    for (int c = 0; c < 9; c++) eps.v[c] = buf_ld(a.eps + c * nipt + blk0, lob) + dEps.v[c];
+    eps = mul(R, mul_bt(eps, R));
+    epl = mul(R, mul_bt(epl, R));
+  }
   T9 Sig;
 #pragma unroll
   for (int k = 0; k < 9; k++) Sig.v[k] = S.v[k] + p * ((k % 4 == 0) ? 1.0 : 0.0);
@@ -474,9 +476,9 @@ __device__ __forceinline__ void elem_hex8_body(const ElemArgs &a, long long blk0
 #pragma unroll
       for (int c = 0; c < 9; c++)
       {
-        (a.eps + blk0 + c * nipt)[lo] = eps.v[c];
-        (a.epsplas + blk0 + c * nipt)[lo] = epl.v[c];
-        (a.deps + blk0 + c * nipt)[lo] = dEps.v[c];
+        buf_st(a.eps + c * nipt + blk0, lob, eps.v[c]);
+        buf_st(a.epsplas + c * nipt + blk0, lob, epl.v[c]);
+        buf_st(a.deps + c * nipt + blk0, lob, dEps.v[c]);
       }
   }
 
