@@ -159,25 +159,32 @@ def test_batch_create_rejects_bad_arguments_without_touching_the_gpu():
     assert L.dnlg_batch_solve(None, 1e-6, 10, None, None) == 1
 
 
-def test_hex8_compile_time_shape_constants_are_the_closed_forms():
-    """The FP64 hex8 kernel fills its shape-function table from constants
-    written as hexadecimal literals; they must be the doubles that the closed
-    forms of FE/ElHex8n3D.C:81-87 round to, which the host computes for the
-    other kernels (IEEE sqrt and division are correctly rounded in Python as in C)."""
-    import math
-    src = open(os.path.join(ROOT, "redynela_amd", "csrc", "dnlg_elem_f64.hpp")).read()
-    lit = dict(re.findall(r"constexpr double (kHexC[012]) = (0x[0-9a-fp.+-]+);", src))
-    assert sorted(lit) == ["kHexC0", "kHexC1", "kHexC2"]
-    assert float.fromhex(lit["kHexC0"]) == (2.0 - math.sqrt(3.0)) / 12.0
-    assert float.fromhex(lit["kHexC1"]) == 1.0 / 12.0
-    assert float.fromhex(lit["kHexC2"]) == (2.0 + math.sqrt(3.0)) / 12.0
-
-
-def test_quad4_compile_time_shape_constants_are_the_closed_forms():
-    """The same for the quad4 table (FE/ElQua4nAx.C:51-73): (3 -+ sqrt 3)/12 and (2 -+ sqrt 3)/6, 1/6."""
-    import math
-    src = open(os.path.join(ROOT, "redynela_amd", "csrc", "dnlg_elem_f64.hpp")).read()
-    lit = {k: float.fromhex(v) for k, v in re.findall(r"constexpr double (kQuad[CN][012]) = (0x[0-9a-fp.+-]+);", src)}
-    r3 = math.sqrt(3.0)
-    assert lit == {"kQuadC0": (3.0 - r3) / 12.0, "kQuadC1": (3.0 + r3) / 12.0, "kQuadN0": (2.0 - r3) / 6.0,
-                   "kQuadN1": 1.0 / 6.0, "kQuadN2": (2.0 + r3) / 6.0}
+def test_compile_time_shape_tables_equal_the_oracle_tables():
+    """The FP64 element kernels copy their shape-function tables from
+    compile-time constants (redynela_amd/csrc/dnlg_tables.hpp), written as
+    hexadecimal literals and expanded by constexpr code. tests/capi/shape_tables
+    prints them; every entry must be the oracle's dN0 (and, for quad4, N) of the
+    same Gauss point and node, bit for bit (the oracle's tables are pinned to
+    the trilinear / bilinear closed forms in test_oracle_element.py)."""
+    import subprocess
+    prog = os.path.join(ROOT, "tests", "capi", "shape_tables")
+    if not os.path.exists(prog):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "tests", "capi"), "shape_tables"])
+    vals = [float.fromhex(t) for t in subprocess.check_output([prog], text=True).split()]
+    assert len(vals) == 196 + 48
+    hexv, quadv = vals[:196], vals[196:]
+    L = O.lib()
+    N8, dN8 = np.zeros((8, 8)), np.zeros((8, 8, 3))
+    L.dnlo_shape_tables(3, N8.reshape(-1), dN8.reshape(-1))
+    for ip in range(8):
+        for k in range(8):
+            for d in range(3):
+                got = hexv[(ip * 8 + k) * 3 + (4 if ip >= 4 else 0) + d]
+                assert got == dN8[ip, k, d] and np.signbit(got) == np.signbit(dN8[ip, k, d]), (ip, k, d)
+    N4, dN4 = np.zeros((4, 4)), np.zeros((4, 4, 2))
+    L.dnlo_shape_tables(2, N4.reshape(-1), dN4.reshape(-1))
+    for ip in range(4):
+        for k in range(4):
+            assert quadv[(ip * 4 + k) * 3 + 0] == dN4[ip, k, 0]
+            assert quadv[(ip * 4 + k) * 3 + 1] == dN4[ip, k, 1]
+            assert quadv[(ip * 4 + k) * 3 + 2] == N4[ip, k]
