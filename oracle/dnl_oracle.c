@@ -65,9 +65,13 @@
  *     (test_oracle_strategies.py);
  *   - HistoryFile points-to-nodes extrapolation: exact on trilinear (hex8) and
  *     bilinear (quad4) fields, node averages (test_oracle_history.py).
+ *   Constant BoundarySpeed (o_apply_constant): the driven component's
+ *   speed, displacement increment (speed x that increment's time step) and
+ *   coordinate update are checked bit for bit at every increment, and the
+ *   undriven components must move (test_deck.py).
  *   Parity unpinned: the Newton iterate sequence for n < 1 (only its
- *   converged consistency is checked), the BoundarySpeed rule, and glibc's
- *   last bits of log / pow (within 1 ulp, test_oracle_math.py).
+ *   converged consistency is checked) and glibc's last bits of log / pow
+ *   (within 1 ulp, test_oracle_math.py).
  */
 #include <math.h>
 #include <stdint.h>

@@ -672,17 +672,39 @@ class Deck:
         v0 = np.zeros((nn, 3))
         smask = np.zeros(nn, np.uint8)
         speed = np.zeros((nn, 3))
+        # Domain::initSolve (FE/Domain.C:1141-1146) applies applyInitial_V0 of a
+        # node's initial conditions, then of its constant ones
+        # (FE/Boundary.C:144-161): BoundarySpeed sets the whole speed vector
+        # (FE/Boundary.C:378-384), BoundaryRestrain nothing (FE/Boundary.C:600-603)
         for v, nodes in self.initial:
             for n in nodes:
                 v0[rank[self.node_index[n]]] = v
         for obj, v, nodes in self.constant:
+            if isinstance(obj, BoundarySpeed):
+                for n in nodes:
+                    v0[rank[self.node_index[n]]] = v
+        # in the time loop BoundarySpeed::applyConstant imposes only the
+        # components whose speed is not zero (FE/Boundary.C:478-524), and a
+        # node's constant conditions apply in the order they were attached
+        # (FE/Boundary.C:200-215); the kernels apply speeds before restraints,
+        # so a speed attached after a restraint on the same component is refused
+        restrained_first = set()
+        for obj, v, nodes in self.constant:
             for n in nodes:
                 i = rank[self.node_index[n]]
                 if isinstance(obj, BoundarySpeed):
-                    smask[i] |= 7
-                    speed[i] = v
+                    for c in range(3):
+                        if v[c] != 0.0:
+                            if (i, c) in restrained_first:
+                                raise DeckError("Structure::attachConstantBC: a BoundarySpeed attached after a "
+                                                "BoundaryRestrain on the same component of node %d is not supported" % n)
+                            smask[i] |= 1 << c
+                            speed[i, c] = v[c]
                 else:
-                    bc[i] |= (1 if v[0] == 1 else 0) | (2 if v[1] == 1 else 0) | (4 if v[2] == 1 else 0)
+                    for c in range(3):
+                        if v[c] == 1:
+                            bc[i] |= 1 << c
+                            restrained_first.add((i, c))
         self._rank = {num: int(rank[self.node_index[num]]) for num in node_nums}
         mesh = dict(dim=self.elem_type.dim, x=x, conn=conn, bc=bc, v0=v0)
         if smask.any():

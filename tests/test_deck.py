@@ -65,7 +65,9 @@ def test_hex_deck():
     assert np.all(m["bc"][m["x"][:, 2] == 0] == 4)
     assert np.all(m["v0"][:, 2] == -150.0)
     top = m["x"][:, 2] == 3e-3
-    assert np.all(m["speed_mask"][top] == 7) and np.all(m["speed_mask"][~top] == 0)
+    # BoundarySpeed (0, 0, -150) imposes z only: applyConstant skips the zero
+    # components (FE/Boundary.C:478-524)
+    assert np.all(m["speed_mask"][top] == 4) and np.all(m["speed_mask"][~top] == 0)
     np.testing.assert_array_equal(m["speed"][top], np.tile([0, 0, -150.0], (top.sum(), 1)))
 
 
@@ -92,6 +94,80 @@ def test_node_and_element_numbering_order():
     assert a["bc"][4] == 1 and a["bc"].sum() == 1
     with pytest.raises(deck.DeckError, match="Node 2 already exists"):
         deck.load("Structure s; s.createNode(2,0,0,0); s.createNode(2,1,0,0);", text=True)
+
+
+_CUBE = """Structure s; ElHex8n3D el; s.setDefaultElement(el);
+s.createNode(1,0,0,0); s.createNode(2,1e-3,0,0); s.createNode(3,1e-3,1e-3,0); s.createNode(4,0,1e-3,0);
+s.createNode(5,0,0,1e-3); s.createNode(6,1e-3,0,1e-3); s.createNode(7,1e-3,1e-3,1e-3); s.createNode(8,0,1e-3,1e-3);
+s.createElement(1,1,2,3,4,5,6,7,8);
+Material m; m.setYoung(206e9); m.setPoisson(.3); m.setDensity(7830);
+ElementSet all; all.add(1); s.attachMaterialToElements(m, all);
+NodeSet base; base.add(1,4); NodeSet top; top.add(5,8);
+ExplicitSolverCH so; so.setTimes(0,5e-6); s.addSolver(so);
+"""
+
+
+def test_constant_speed_imposes_nonzero_components_and_initial_velocity():
+    """a constant BoundarySpeed with no initial condition: applyInitial_V0 of the
+    constant conditions (FE/Boundary.C:144-161, 378-384) starts the driven nodes
+    at its whole speed vector, and applyConstant (FE/Boundary.C:478-524) imposes
+    only its nonzero components; the others stay free"""
+    from redynela_amd import deck
+    import oracle_lib as O
+    d = deck.load(_CUBE + """BoundaryRestrain bz; bz.set(0,0,1); s.attachConstantBC(bz, base);
+    BoundarySpeed vt; vt.set(0, 0, -20); s.attachConstantBC(vt, top);""", text=True)
+    m, _, _ = d.model()
+    top = m["x"][:, 2] > 0
+    np.testing.assert_array_equal(m["v0"][top], np.tile([0, 0, -20.0], (4, 1)))
+    assert not m["v0"][~top].any()
+    assert np.all(m["speed_mask"][top] == 4) and not m["speed_mask"][~top].any()
+    assert np.all(m["bc"][~top] == 4) and not m["bc"][top].any()
+    o = O.from_deck(d)
+    assert o.begin(1e300) == 1
+    x0 = m["x"].copy()
+    xp = o.get(O.F_X).reshape(-1, 3)
+    k = 0
+    while True:
+        dt = o.time()["dt"]
+        r = o.step()
+        k += 1
+        v = o.get(O.F_V).reshape(-1, 3)
+        du = o.get(O.F_DU).reshape(-1, 3)
+        x = o.get(O.F_X).reshape(-1, 3)
+        # the driven component: speed exactly the prescribed one, displacement
+        # increment exactly speed x the increment's time step, coordinates
+        # advanced by it; the restrained base does not move
+        assert np.all(v[top, 2] == -20.0) and np.all(v[~top, 2] == 0.0)
+        assert np.all(du[top, 2] == -20.0 * dt) and np.all(du[~top, 2] == 0.0)
+        assert np.array_equal(x[:, 2], xp[:, 2] + du[:, 2])
+        xp = x
+        if r != 1:
+            break
+    assert k >= 20
+    x = o.get(O.F_X).reshape(-1, 3)
+    # the block is squeezed along z, so the free x / y components of the driven
+    # nodes bulge out: a mask of 7 would have held them at zero speed
+    assert np.all(np.abs(x[top, :2] - x0[top, :2]) > 0)
+    assert np.all(x[top, 2] < x0[top, 2])
+
+
+def test_speed_after_restrain_on_one_component_raises():
+    """constant conditions of a node apply in the order they were attached
+    (FE/Boundary.C:200-215); a speed attached after a restraint of the same
+    component would win in the reference, which this reader refuses"""
+    from redynela_amd import deck
+    with pytest.raises(deck.DeckError, match="BoundarySpeed attached after a BoundaryRestrain"):
+        deck.load(_CUBE + """BoundaryRestrain bz; bz.set(0,0,1); s.attachConstantBC(bz, top);
+        BoundarySpeed vt; vt.set(0, 0, -20); s.attachConstantBC(vt, top);""", text=True).model()
+    # restraint after speed: the restraint wins, as in the reference
+    m, _, _ = deck.load(_CUBE + """BoundarySpeed vt; vt.set(0, 0, -20); s.attachConstantBC(vt, top);
+    BoundaryRestrain bz; bz.set(0,0,1); s.attachConstantBC(bz, top);""", text=True).model()
+    top = m["x"][:, 2] > 0
+    assert np.all(m["bc"][top] == 4) and np.all(m["speed_mask"][top] == 4)
+    # a speed on x after a restraint on z touches another component: accepted
+    m, _, _ = deck.load(_CUBE + """BoundaryRestrain bz; bz.set(0,0,1); s.attachConstantBC(bz, top);
+    BoundarySpeed vx; vx.set(5, 0, 0); s.attachConstantBC(vx, top);""", text=True).model()
+    assert np.all(m["speed_mask"][top] == 1) and np.all(m["bc"][top] == 4)
 
 
 def test_expression_rules():
