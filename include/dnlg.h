@@ -78,7 +78,11 @@ typedef struct
 /* boundary conditions: BoundaryRestrain (FE/Boundary.C:636-666) as a per-node
  * component mask (bit c = component c held), initial BoundarySpeed
  * (FE/Boundary.C:378-384) as a per-node velocity, and an optional constant
- * BoundarySpeed (FE/Boundary.C:478-524): mask + prescribed velocity. */
+ * BoundarySpeed (FE/Boundary.C:478-524): mask + prescribed velocity.  The
+ * reference drives only a speed's nonzero components, so bit c of speed_mask
+ * is set where speed[c] != 0.  Speeds are applied before restraints: a
+ * component both driven and held ends held, as when the reference attaches
+ * the restraint last. */
 typedef struct
 {
   const unsigned char *restrain;   /* [n_nodes] or NULL */
