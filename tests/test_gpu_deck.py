@@ -87,6 +87,32 @@ def test_deck_result_files_match_oracle(tmp_path, name):
     assert np.all(RF.read_plot(plot)[:, 3] > 0)
 
 
+def test_hex_deck_two_component_speed_matches_oracle(tmp_path):
+    """the hex deck with its top face driven at (20, 0, -150) m/s: speed mask 5
+    (x and z driven, y free, FE/Boundary.C:478-524); every result file and the
+    final state equal the oracle's bit for bit"""
+    from redynela_amd import deck
+    say("case: hex deck, constant BoundarySpeed on x and z (mask 5), GPU FP64 against the oracle")
+    text = open(os.path.join(HERE, "decks", "hex_block.dyn")).read()
+    assert text.count("vt.set(0, 0, -speed);") == 1
+    path = tmp_path / "hex_block_xz.dyn"
+    path.write_text(text.replace("vt.set(0, 0, -speed);", "vt.set(20, 0, -speed);"))
+    d = deck.load(str(path))
+    m, _, _ = d.model()
+    top = m["x"][:, 2] == 3e-3
+    assert np.all(m["speed_mask"][top] == 5) and np.all(m["v0"][top] == [20.0, 0.0, -150.0])
+    dom = d.run(results=str(tmp_path / "gpu" / "hex_block_xz"))
+    gpu = sorted(f for f in os.listdir(tmp_path / "gpu") if f.endswith(".bin"))
+    ref, _ = oracle_files(d, str(tmp_path / "hex_block_xz"))
+    assert gpu == [os.path.basename(p) for p in ref] and len(gpu) >= 2
+    for g, r in zip(gpu, ref):
+        assert open(tmp_path / "gpu" / g, "rb").read() == open(r, "rb").read(), g
+    x = dom.coords.reshape(-1, 3)
+    # the driven x and z moved at their speeds; y of the top face is free
+    assert np.all(x[top, 0] > m["x"][top, 0]) and np.all(x[top, 2] < m["x"][top, 2])
+    dom.close()
+
+
 def test_plot_rows_per_increment():
     """the history lines against the oracle's endStep values, per increment:
     time and time steps bit-equal, kinetic energy to 1e-13"""
