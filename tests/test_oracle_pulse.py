@@ -155,3 +155,21 @@ def test_power_iteration_converges_to_the_largest_eigenvalue(name):
     assert abs(f * f - lam[0]) / lam[0] < 1e-8
     # the first call stops within 1 % of it (the reference's 1e-4 test on successive fmax)
     assert abs(f1 * f1 - lam[0]) / lam[0] < 0.02
+
+
+@pytest.mark.parametrize("hz, lo, hi", [(1.0e-3, 1.30, 1.40), (0.4e-3, 1.00, 1.06), (0.0648e-3, 1.000, 1.005)])
+def test_hex8_courant_length_against_the_top_frequency(hz, lo, hi):
+    """the build-defined hex8 Courant length V / A_max against the true top
+    frequency of one fully integrated element with lumped mass (eigenvalues
+    of M^-1 K, numpy): flat elements (the bar's 64.8 um layers, 0.4 aspect
+    plates) stay inside the 0.9 safety factor, a cube does not"""
+    h = 1e-3
+    x = np.array([(0, 0, 0), (h, 0, 0), (h, h, 0), (0, h, 0), (0, 0, hz), (h, 0, hz), (h, h, hz), (0, h, hz)], float)
+    o = one_element(3, x)
+    mesh = dict(dim=3, x=x, conn=np.arange(8, dtype=np.int32)[None, :])
+    o.compute_internal_matrices()
+    f_courant = 2.0 / o.courant_crit()  # f_max = 2 / min(l / c), FE/Solver.C:220-267
+    o.max_frequency()  # lumped mass
+    ratio = np.sqrt(assembled(o, mesh)[0]) / f_courant
+    assert lo < ratio < hi, ratio
+    assert (ratio * 0.9 < 1.0) == (hz < h)  # the 0.9 safety factor of the time step
