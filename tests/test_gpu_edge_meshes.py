@@ -26,8 +26,8 @@ def hex_column(n, h=1e-3, hz=0.4e-3, v=-150.0, shear=20.0):
     base face counter-clockwise, then the top face), base restrained along z,
     every node at v along z, with an x shear so that the sweep rotates.  The
     plates are thinner than wide, as the O-grid bar's layers are: the hex8
-    Courant length V / A_max then bounds the element's highest frequency (for
-    a cube it does not, and the central-difference step is unstable)"""
+    Courant length V / A_max then bounds the element's highest frequency (for one
+    lone cube it does not, and the central-difference step is unstable)"""
     x = []
     for k in range(n + 1):
         for (i, j) in ((0, 0), (1, 0), (1, 1), (0, 1)):

@@ -157,19 +157,36 @@ def test_power_iteration_converges_to_the_largest_eigenvalue(name):
     assert abs(f1 * f1 - lam[0]) / lam[0] < 0.02
 
 
-@pytest.mark.parametrize("hz, lo, hi", [(1.0e-3, 1.30, 1.40), (0.4e-3, 1.00, 1.06), (0.0648e-3, 1.000, 1.005)])
-def test_hex8_courant_length_against_the_top_frequency(hz, lo, hi):
+def hex_grid(n, nz, h, hz):
+    """n x n x nz regular hex8 grid (ElHex8n3D node order), free"""
+    idx, x, conn = {}, [], []
+    for k in range(nz + 1):
+        for j in range(n + 1):
+            for i in range(n + 1):
+                idx[(i, j, k)] = len(x)
+                x.append((i * h, j * h, k * hz))
+    for k in range(nz):
+        for j in range(n):
+            for i in range(n):
+                c = [(i, j, k), (i + 1, j, k), (i + 1, j + 1, k), (i, j + 1, k),
+                     (i, j, k + 1), (i + 1, j, k + 1), (i + 1, j + 1, k + 1), (i, j + 1, k + 1)]
+                conn.append([idx[t] for t in c])
+    x = np.array(x)
+    return dict(dim=3, x=x, conn=np.array(conn, np.int32), bc=np.zeros(len(x), np.uint8), v0=np.zeros_like(x))
+
+
+@pytest.mark.parametrize("n, nz, hz, lo, hi", [(1, 1, 1.0e-3, 1.30, 1.40), (4, 4, 1.0e-3, 1.02, 1.06),
+                                               (4, 6, 0.4e-3, 0.99, 1.01), (4, 6, 0.5e-3, 0.99, 1.01)])
+def test_hex8_courant_length_against_the_top_frequency(n, nz, hz, lo, hi):
     """the build-defined hex8 Courant length V / A_max against the true top
-    frequency of one fully integrated element with lumped mass (eigenvalues
-    of M^-1 K, numpy): flat elements (the bar's 64.8 um layers, 0.4 aspect
-    plates) stay inside the 0.9 safety factor, a cube does not"""
-    h = 1e-3
-    x = np.array([(0, 0, 0), (h, 0, 0), (h, h, 0), (0, h, 0), (0, 0, hz), (h, 0, hz), (h, h, hz), (0, h, hz)], float)
-    o = one_element(3, x)
-    mesh = dict(dim=3, x=x, conn=np.arange(8, dtype=np.int32)[None, :])
+    frequency of the assembled mesh (eigenvalues of M^-1 K with lumped mass,
+    numpy): regular meshes of cubes and of flat plates stay inside the 0.9
+    safety factor of the time step; one lone cube does not"""
+    mesh = hex_grid(n, nz, 1e-3, hz)
+    o = O.Model(mesh, STEEL, dict(ts_method="Pulse"))
     o.compute_internal_matrices()
     f_courant = 2.0 / o.courant_crit()  # f_max = 2 / min(l / c), FE/Solver.C:220-267
     o.max_frequency()  # lumped mass
     ratio = np.sqrt(assembled(o, mesh)[0]) / f_courant
     assert lo < ratio < hi, ratio
-    assert (ratio * 0.9 < 1.0) == (hz < h)  # the 0.9 safety factor of the time step
+    assert (ratio * 0.9 < 1.0) == (n > 1)
