@@ -6,10 +6,11 @@ namespace dnl {
 
 __global__ __launch_bounds__(128) void k_elem_hex8_f32(ElemArgs32 a)
 {
+  __shared__ float lx[48 * 128];  // each lane's element node rows, 24 KB: 3 waves per SIMD
   if ((a.check_running && !a.st->running) || a.st->halted) return;
   const long long e = (long long)blockIdx.x * 128 + threadIdx.x;
   if (e >= a.ne) return;
-  elem_hex8_f32_body(a, e);
+  elem_hex8_f32_body<128>(a, e, lx + threadIdx.x);
 }
 
 __global__ __launch_bounds__(128) void k_elem_quad4ax_f32(ElemArgs32 a)
