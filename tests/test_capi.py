@@ -188,3 +188,22 @@ def test_compile_time_shape_tables_equal_the_oracle_tables():
             assert quadv[(ip * 4 + k) * 3 + 0] == dN4[ip, k, 0]
             assert quadv[(ip * 4 + k) * 3 + 1] == dN4[ip, k, 1]
             assert quadv[(ip * 4 + k) * 3 + 2] == N4[ip, k]
+
+
+def test_missing_library_fails_loudly():
+    """without the built HIP library every entry point raises: there is no
+    CPU path behind the Domain API (run in a child process, so that this
+    process keeps its loaded library)"""
+    import subprocess
+    import sys
+    code = ("import redynela_amd as R\n"
+            "from redynela_amd import _lib as L\n"
+            "L.LIB_PATH = '/nonexistent/libdnlgpu.so'\n"
+            "m = R.Material(115e9, 0.3, 8930.0, R.IsoHardElastoplastic(400e6, 100e6, 1.0))\n"
+            "s = R.ExplicitSolverCH()\n"
+            "try:\n"
+            "    R.Domain(R.mesh.taylor_axi(2, 4), m, s)\n"
+            "except L.DnlgError as e:\n"
+            "    print('raised:', e)\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert "raised: libdnlgpu.so is not built" in r.stdout, (r.stdout, r.stderr)
