@@ -450,7 +450,11 @@ __device__ __forceinline__ void elem_hex8_f32_body(const ElemArgs32 &a, long lon
 // ---------------------------------------------------------------------------
 // axisymmetric quad4, one element per lane
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void elem_quad4ax_f32_body(const ElemArgs32 &a, long long e)
+// NT = 0: the element's node rows (coordinates relative to node 0, radius,
+// Current increment: 20 floats) in registers; NT > 0: in the lane's own LDS
+// column lx[i * NT], read again at every Gauss point (as the hex8 body does)
+template <int NT = 0>
+__device__ __forceinline__ void elem_quad4ax_f32_body(const ElemArgs32 &a, long long e, float *lx = nullptr)
 {
   StepState *st = a.st;
   const long long ne = a.ne;
@@ -458,17 +462,22 @@ __device__ __forceinline__ void elem_quad4ax_f32_body(const ElemArgs32 &a, long 
   int nd[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) nd[k] = gmem(a.conn)[e * 4 + k];
-  float xr[4][2], r[4], u[4][2];
+  float rreg[NT == 0 ? 20 : 1];
+  // row 5k + c: x_c - x_c(node 0); 5k + 2: radius; 5k + 3 + c: increment
+  auto row = [&](int i) -> float & {
+    if constexpr (NT == 0) return rreg[i];
+    else return lx[i * NT];
+  };
   const double x00 = gmem(a.x)[3 * (long long)nd[0]], x01 = gmem(a.x)[3 * (long long)nd[0] + 1];
 #pragma unroll
   for (int k = 0; k < 4; k++)
   {
     const auto *xk = gmem(a.x) + 3 * (long long)nd[k];
-    xr[k][0] = (float)(xk[0] - x00);
-    xr[k][1] = (float)(xk[1] - x01);
-    r[k] = (float)xk[0];
-    u[k][0] = (float)gmem(a.du)[3 * (long long)nd[k]];
-    u[k][1] = (float)gmem(a.du)[3 * (long long)nd[k] + 1];
+    row(5 * k + 0) = (float)(xk[0] - x00);
+    row(5 * k + 1) = (float)(xk[1] - x01);
+    row(5 * k + 2) = (float)xk[0];
+    row(5 * k + 3) = (float)gmem(a.du)[3 * (long long)nd[k]];
+    row(5 * k + 4) = (float)gmem(a.du)[3 * (long long)nd[k] + 1];
   }
   float Af[4][2], g[4][2];
 #pragma unroll
@@ -479,16 +488,18 @@ __device__ __forceinline__ void elem_quad4ax_f32_body(const ElemArgs32 &a, long 
 #pragma unroll
   for (int pt = 0; pt < 4; pt++)
   {
+    if constexpr (NT > 0) __asm__ volatile("" ::: "memory");  // keep the LDS rows out of registers
     float J0 = 0, J1 = 0, J2 = 0, J3 = 0, Rr = 0.f, ur = 0.f;
 #pragma unroll
     for (int k = 0; k < 4; k++)
     {
       float d0[2], N;
       quad_dn0f(pt, k, d0, &N);
-      J0 += d0[0] * xr[k][0]; J1 += d0[1] * xr[k][0];
-      J2 += d0[0] * xr[k][1]; J3 += d0[1] * xr[k][1];
-      Rr += N * r[k];
-      ur += N * u[k][0];
+      const float x0 = row(5 * k), x1 = row(5 * k + 1);
+      J0 += d0[0] * x0; J1 += d0[1] * x0;
+      J2 += d0[0] * x1; J3 += d0[1] * x1;
+      Rr += N * row(5 * k + 2);
+      ur += N * row(5 * k + 3);
     }
     const float detJ = J0 * J3 - J1 * J2;
     const float id = 1.f / detJ;
@@ -500,8 +511,9 @@ __device__ __forceinline__ void elem_quad4ax_f32_body(const ElemArgs32 &a, long 
       float d0[2], N;
       quad_dn0f(pt, k, d0, &N);
       const float d[2] = {d0[0] * inv[0] + d0[1] * inv[2], d0[0] * inv[1] + d0[1] * inv[3]};
-      H[0] += u[k][0] * d[0]; H[1] += u[k][0] * d[1];
-      H[3] += u[k][1] * d[0]; H[4] += u[k][1] * d[1];
+      const float u0 = row(5 * k + 3), u1 = row(5 * k + 4);
+      H[0] += u0 * d[0]; H[1] += u0 * d[1];
+      H[3] += u1 * d[0]; H[4] += u1 * d[1];
     }
     H[8] = ur / Rr;
     float dE[6], R[9];
