@@ -28,15 +28,34 @@ __device__ __forceinline__ int s0(int k) { int q = k & 3; return (q == 1 || q ==
 __device__ __forceinline__ int s1(int k) { return ((k & 3) >= 2) ? 1 : -1; }
 __device__ __forceinline__ int s2(int k) { return (k >= 4) ? 1 : -1; }
 
-// trilinear / bilinear shape-function derivatives at the 2-point Gauss points
-__device__ __forceinline__ void hex_dn0f(int ip, int k, float *g)
+// trilinear shape-function derivatives dN0(ip, k, d) of hex8 at its eight
+// 2-point Gauss points, a compile-time table in constant memory: the point
+// loop indexes it with a wave-uniform point, so the values arrive by scalar
+// loads instead of being formed per lane
+struct HexTab32
 {
+  float v[8 * 8 * 3];
+};
+__host__ __device__ constexpr HexTab32 make_hex_tab32()
+{
+  HexTab32 t{};
   const float r3 = 0.57735026918962576f;  // 1/sqrt(3)
-  const float a0 = 1.f + s0(k) * s0(ip) * r3, a1 = 1.f + s1(k) * s1(ip) * r3, a2 = 1.f + s2(k) * s2(ip) * r3;
-  g[0] = 0.125f * s0(k) * a1 * a2;
-  g[1] = 0.125f * s1(k) * a0 * a2;
-  g[2] = 0.125f * s2(k) * a0 * a1;
+  for (int ip = 0; ip < 8; ip++)
+    for (int k = 0; k < 8; k++)
+    {
+      const int q = k & 3, qi = ip & 3;
+      const float sk0 = (q == 1 || q == 2) ? 1.f : -1.f, sk1 = q >= 2 ? 1.f : -1.f, sk2 = k >= 4 ? 1.f : -1.f;
+      const float si0 = (qi == 1 || qi == 2) ? 1.f : -1.f, si1 = qi >= 2 ? 1.f : -1.f, si2 = ip >= 4 ? 1.f : -1.f;
+      const float a0 = 1.f + sk0 * si0 * r3, a1 = 1.f + sk1 * si1 * r3, a2 = 1.f + sk2 * si2 * r3;
+      t.v[(ip * 8 + k) * 3 + 0] = 0.125f * sk0 * a1 * a2;
+      t.v[(ip * 8 + k) * 3 + 1] = 0.125f * sk1 * a0 * a2;
+      t.v[(ip * 8 + k) * 3 + 2] = 0.125f * sk2 * a0 * a1;
+    }
+  return t;
 }
+static __constant__ HexTab32 kHexTab32 = make_hex_tab32();
+
+// bilinear shape functions and derivatives of quad4 at its 2-point Gauss points
 __device__ __forceinline__ void quad_dn0f(int ip, int k, float *g, float *N)
 {
   const float r3 = 0.57735026918962576f;
@@ -371,7 +390,12 @@ __device__ __forceinline__ void elem_hex8_f32_body(const ElemArgs32 &a, long lon
     // dN0 of the 8 nodes at this point, then (in place) dN = dN0 J^-1 for H and B^T sigma
     float dn[8][3];
 #pragma unroll
-    for (int k = 0; k < 8; k++) hex_dn0f(pt, k, dn[k]);
+    for (int k = 0; k < 8; k++)
+    {
+      dn[k][0] = kHexTab32.v[(pt * 8 + k) * 3 + 0];
+      dn[k][1] = kHexTab32.v[(pt * 8 + k) * 3 + 1];
+      dn[k][2] = kHexTab32.v[(pt * 8 + k) * 3 + 2];
+    }
     float J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int k = 1; k < 8; k++)
