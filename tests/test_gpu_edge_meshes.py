@@ -74,3 +74,34 @@ def test_edge_mesh_per_step(name):
     assert same(g.field(L.F_V), o.get(O.F_V))
     assert np.max(o.get(O.F_EVP)) > 1e-4, "no plastic flow: the radial return did not run"
     g.close()
+
+
+@pytest.mark.parametrize("name", ["hex8 x1", "hex8 x33", "quad4 axi x1"])
+def test_edge_mesh_mixed_form_tracks_fp64(name):
+    """the mixed form on the same meshes (its hex8 pass keeps each lane's node
+    rows in LDS; lanes past the last element return before touching it):
+    after 150 increments the displacement field is within 1e-4 of FP64,
+    the final-shape bar"""
+    import redynela_amd as R
+    from redynela_amd import _lib as L
+    mesh = {"hex8 x1": lambda: hex_column(1, v=-20.0, shear=5.0), "hex8 x33": lambda: hex_column(33),
+            "quad4 axi x1": quad_single}[name]()
+    say("case: %s, mixed form against FP64 on the GPU, 150 increments" % name)
+    out = {}
+    for form in ("fp64", "mixed"):
+        m = R.Material(COPPER["E"], COPPER["nu"], COPPER["rho"],
+                       R.IsoHardElastoplastic(COPPER["A"], COPPER["B"], COPPER["n"]))
+        s = R.ExplicitSolverCH()
+        s.setTimeStepMethod("Courant")
+        s.setDissipation(1.0)
+        s.setPrecision(form)
+        g = R.Domain(mesh, m, s)
+        g.begin(1.0)
+        g.step(150)
+        out[form] = (g.field(L.F_COORDS).reshape(-1, 3), g.time()["increment"])
+        g.close()
+    x0 = np.asarray(mesh["x"], dtype=np.float64)
+    d64, dmx = out["fp64"][0] - x0, out["mixed"][0] - x0
+    err = np.max(np.abs(dmx - d64)) / np.max(np.abs(d64))
+    say("  displacement error %.3e" % err)
+    assert out["fp64"][1] == out["mixed"][1] and err < 1e-4
