@@ -105,3 +105,23 @@ def test_edge_mesh_mixed_form_tracks_fp64(name):
     err = np.max(np.abs(dmx - d64)) / np.max(np.abs(d64))
     say("  displacement error %.3e" % err)
     assert out["fp64"][1] == out["mixed"][1] and err < 1e-4
+
+
+def test_single_cube_pulse_per_step():
+    """one 1 mm cube under the Pulse time step, the reference's hex8 method:
+    GPU against the oracle bit for bit per increment (time scalars,
+    coordinates, iteration counts of the power iteration), and bounded,
+    where the build-defined Courant length would be unstable on a lone cube"""
+    from redynela_amd import _lib as L
+    from test_gpu_pulse import check, pair
+    mesh = hex_column(1, hz=1e-3, v=-10.0, shear=2.0)
+    say("case: one 1 mm hex8 cube, Pulse time step, 120 increments, GPU against the oracle")
+    g, o = pair(mesh, COPPER)
+    assert g.begin(1.0) and o.begin(1.0) == 1
+    check(g, o, "cube preamble")
+    for k in range(120):
+        g.step(1)
+        assert o.step() >= 0
+        check(g, o, "cube step %d" % (k + 1))
+    assert np.max(np.abs(g.field(L.F_V))) < 50.0  # no blow-up
+    g.close()
