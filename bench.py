@@ -28,6 +28,9 @@ sys.path.insert(0, ROOT)
 #   connectivity 8 x 4 B = 32 B
 #   nodal traffic 276 B per node x (nodes / elements)
 HIST_B, CONN_B, NODE_B, ELEM_NODE_READ_B = 1024.0, 32.0, 276.0, 48.0
+# the mixed form's element pass: FP32 point history, connectivity, FP64 node
+# reads and FP32 force slots, 798 B per element
+MIXED_ELEM_B = 798.0
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 VALU_PEAK_GINST = 256 * 4 * 2.4 / 4  # wave64 VALU instructions per ns: 256 CUs x 4 SIMDs, 4 cycles each at 2.4 GHz
@@ -202,14 +205,24 @@ def mixed_form(mesh, args, fp64_value, fp64_state=None):
     assert dom.begin(1.0)
     dom.step_async(args.warmup)
     dom.synchronize()
+    dom.enable_kernel_timing(True)  # HIP events around the element launches, as in the FP64 window
     t0 = time.perf_counter()
     dom.step_async(args.steps)
     dom.synchronize()
     el = time.perf_counter() - t0
+    n_el, ms_el = dom.kernel_stats(0)
+    dom.enable_kernel_timing(False)
     ne = mesh["conn"].shape[0]
     v = ne * args.steps / el
+    avg_el = ms_el / max(n_el, 1) * 1e-3
+    achieved = MIXED_ELEM_B * ne / avg_el / 1e9
     out = {"value": v, "unit": "element-steps/s", "ms_per_step": el / args.steps * 1e3,
            "speedup_vs_fp64": v / fp64_value,
+           "element_kernel_ms": avg_el * 1e3,
+           "roofline": {"bound": "hbm", "kernel": "k_elem_hex8_f32 (FP32 element pass)", "achieved": achieved,
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                        "algorithmic_bytes_per_element": MIXED_ELEM_B,
+                        "counters": "profiles/k_elem_hex8_f32_1M.md"},
            "note": "same mesh, steps and warm-up; FP32 kinematics / stress / B^T sigma, FP64 nodal state and time"}
     if fp64_state is not None:
         x64, e64 = fp64_state
