@@ -207,3 +207,13 @@ def test_missing_library_fails_loudly():
             "    print('raised:', e)\n")
     r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
     assert "raised: libdnlgpu.so is not built" in r.stdout, (r.stdout, r.stderr)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; a request for more exits with a message,
+    before anything touches a device"""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], cwd=ROOT,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "--gpus 2 is not supported" in (r.stdout + r.stderr)
